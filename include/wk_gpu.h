@@ -1,0 +1,296 @@
+/*
+ * wk_gpu.h -- C ABI of the MI355X-native Wukong GPU query path.
+ *
+ * This is the drop-in boundary that replaces the reference's GPU extension
+ * (src/gpu/).  Every entry point takes plain pointers and sizes; no C++ or
+ * HIP types cross it.  A handle is used from one host thread at a time;
+ * independent handles are independent.  All calls return a status code
+ * (0 = WK_OK); nothing aborts.  Tables are row-major uint32 `[rows][cols]`,
+ * the reference's history-table layout (core/sparql/query.hpp:392-397).
+ *
+ * Reference interface each group replaces:
+ *   store   : SegmentRDFGraph::init_gstore / get_triples / get_index
+ *             (core/store/segment_rdf_dgraph.hpp:1114-1129,1182-1269)
+ *   device  : GPUMem + GPUCache + GPUStreamPool set-up in main()
+ *             (src/wukong.cpp:182-190,280-295; src/gpu/gpu_mem.hpp:38-139;
+ *             src/gpu/gpu_cache.hpp:428-470)
+ *   segment : GPUCache::load_segment / prefetch_segment
+ *             (src/gpu/gpu_cache.hpp:551-559)
+ *   table   : GPUEngine::load_result_buf, GPUEngineCuda::load_result_buf
+ *             and the final thrust::copy D2H
+ *             (src/gpu/gpu_engine.hpp:248-263; src/gpu/gpu_engine_cuda.hpp:90-106,190-193)
+ *   pattern : GPUEngineCuda::known_to_unknown / known_to_known / known_to_const
+ *             (src/gpu/gpu_engine_cuda.hpp:113-198,200-282,284-363) and the
+ *             CPU-side starts GPUEngine::index_to_unknown / const_to_unknown
+ *             (src/gpu/gpu_engine.hpp:65-125), which run on the device here
+ *   query   : GPUEngine::execute_one_pattern looped by
+ *             GPUAgent::execute_sparql_query (src/gpu/gpu_engine.hpp:265-338;
+ *             src/gpu/gpu_agent.hpp:173-246)
+ */
+#ifndef WK_GPU_H
+#define WK_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  0..13 keep the numbering of core/common/errors.hpp:30-45 so
+ * a reply's status_code means the same thing on both engines. */
+enum {
+  WK_OK = 0,
+  WK_E_UNKNOWN = 1,
+  WK_E_SYNTAX = 2,
+  WK_E_UNKNOWN_PATTERN = 3,
+  WK_E_ATTR_DISABLE = 4,
+  WK_E_NO_REQUIRED_VAR = 5,
+  WK_E_UNSUPPORT_UNION = 6,
+  WK_E_OBJ_ERROR = 7,
+  WK_E_VERTEX_INVALID = 8,
+  WK_E_UNKNOWN_SUB = 9,
+  WK_E_SETTING = 10,
+  WK_E_FIRST_PATTERN = 11,
+  WK_E_UNKNOWN_FILTER = 12,
+  WK_E_FILE_NOT_FOUND = 13,
+  /* codes of this library (no counterpart in the reference, which aborts) */
+  WK_E_INVALID_ARG = 100,     /* null handle, bad column, bad direction ...  */
+  WK_E_INVALID_TRIPLE = 101,  /* triple violates the id conventions          */
+  WK_E_NOMEM_HOST = 102,      /* host allocation failed                      */
+  WK_E_NOMEM_BUDGET = 103,    /* device budget (gstore or tables) exceeded   */
+  WK_E_HIP = 104,             /* a HIP runtime call failed; see wk_last_hip_error */
+  WK_E_NO_DEVICE = 105,       /* no usable GPU                               */
+  WK_E_OVERFLOW = 106,        /* a count exceeds the 64-bit/28-bit fields    */
+  WK_E_SEG_TOO_LARGE = 107,   /* one segment exceeds the gstore budget       */
+  WK_E_STATE = 108,           /* call out of order (e.g. no result yet)      */
+  WK_E_TOO_MANY_VARS = 109    /* more variables than WK_MAX_VARS             */
+};
+
+#define WK_IN 0
+#define WK_OUT 1
+#define WK_PREDICATE_ID 0
+#define WK_TYPE_ID 1
+#define WK_MAX_VARS 64
+
+typedef struct wk_store wk_store;
+typedef struct wk_dev wk_dev;
+typedef struct wk_table wk_table;
+
+/* A triple pattern, SPARQLQuery::Pattern (core/sparql/query.hpp:145-170):
+ * negative ids are variables -1, -2, ...; direction is WK_IN or WK_OUT. */
+typedef struct {
+  int32_t subject;
+  int32_t predicate;
+  int32_t object;
+  int32_t direction;
+} wk_pattern;
+
+/* A BASIC pattern group already in plan order (after the .fmt rewrite of
+ * Planner::set_direction, optimizer/planner.hpp:667-699). */
+typedef struct {
+  const wk_pattern* patterns;
+  uint32_t npatterns;
+  int32_t nvars;
+} wk_query;
+
+/* ------------------------------------------------------------------ store */
+
+typedef struct {
+  int32_t pinned;          /* 1: allocate slots/values in pinned host memory */
+  int32_t load_factor_pct; /* #keys/#slots target; <=0 -> 55 (global.hpp:127) */
+} wk_store_opts;
+
+typedef struct {
+  uint64_t num_triples_out; /* distinct (s,p,o) after the pso sort+dedup */
+  uint64_t num_triples_in;  /* distinct (s,p,o) after the pos sort+dedup */
+  uint64_t num_keys;
+  uint64_t num_buckets;     /* main + extended, all segments */
+  uint64_t num_values;      /* edges, all segments */
+  uint32_t num_segments;
+  int32_t pinned;
+} wk_store_info;
+
+typedef struct {
+  uint32_t index;           /* 1 = index segment */
+  uint32_t pid;
+  uint32_t dir;
+  uint32_t pad;
+  uint64_t num_keys;
+  uint64_t num_buckets;     /* main buckets: hash space of the segment */
+  uint64_t num_ext_buckets; /* chained buckets after the main ones */
+  uint64_t bucket_start;    /* global bucket offset of bucket 0 */
+  uint64_t num_edges;
+  uint64_t edge_start;      /* global edge offset of the segment's values */
+} wk_seg_info;
+
+/* SegmentRDFGraph::init_gstore(triple_pso, triple_pos, ...)
+ * (core/store/segment_rdf_dgraph.hpp:1182-1269).  `pso` and `pos` are flat
+ * arrays of (s,p,o) uint32 triples; they may be the same array. */
+int wk_store_build(const uint32_t* pso, uint64_t n_pso, const uint32_t* pos, uint64_t n_pos,
+                   const wk_store_opts* opts, wk_store** out);
+void wk_store_destroy(wk_store* st);
+int wk_store_info_get(const wk_store* st, wk_store_info* info);
+int wk_store_segment(const wk_store* st, uint32_t i, wk_seg_info* seg);
+/* Raw arrays: `slots` = 16-byte {key,ptr} pairs, 8 per bucket. */
+int wk_store_raw(const wk_store* st, const void** slots, uint64_t* num_buckets,
+                 const uint32_t** values, uint64_t* num_values);
+/* DGraph::get_triples / get_index (segment_rdf_dgraph.hpp:1114-1129). */
+int wk_store_get_triples(const wk_store* st, uint32_t vid, uint32_t pid, int32_t dir,
+                         const uint32_t** edges, uint64_t* size);
+int wk_store_get_index(const wk_store* st, uint32_t tpid, int32_t dir,
+                       const uint32_t** edges, uint64_t* size);
+
+/* ----------------------------------------------------------------- device */
+
+typedef struct {
+  int32_t device;             /* HIP device ordinal */
+  uint64_t gstore_budget;     /* bytes of HBM for the gstore cache; 0 -> whole store */
+  uint64_t table_budget;      /* bytes for history tables + scratch; 0 -> 8 GiB */
+  uint32_t key_block_shift;   /* log2 buckets per key block; 0 -> 14 (2 MiB) */
+  uint32_t value_block_shift; /* log2 edges per value block; 0 -> 19 (2 MiB) */
+  int32_t enable_pipeline;    /* prefetch next pattern's segment (global.hpp:136) */
+  int32_t probe_width;        /* lanes per key in the probe: 1,2,4,8; 0 -> 8 */
+  int32_t rows_per_wave;      /* gather mapping: 1,4,16,64 or 0 = balanced search */
+  int32_t profile;            /* 1: time every kernel with HIP events */
+} wk_dev_opts;
+
+typedef struct {
+  uint64_t h2d_bytes;         /* gstore bytes swapped in */
+  uint64_t segments_loaded;
+  uint64_t segments_evicted;
+  uint64_t key_blocks_total, key_blocks_free;
+  uint64_t value_blocks_total, value_blocks_free;
+  uint64_t table_capacity_elems;
+  uint64_t queries_run;
+} wk_dev_stats;
+
+/* Per-kernel timing (opts.profile = 1): name, launches, total ms on the
+ * library's stream, algorithmic bytes moved. */
+typedef struct {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+  double alg_bytes;
+} wk_kernel_stat;
+
+int wk_dev_create(const wk_store* st, const wk_dev_opts* opts, wk_dev** out);
+void wk_dev_destroy(wk_dev* dev);
+int wk_dev_stats_get(wk_dev* dev, wk_dev_stats* stats);
+int wk_dev_kernel_stats(wk_dev* dev, wk_kernel_stat* out, uint32_t cap, uint32_t* n);
+int wk_dev_reset_kernel_stats(wk_dev* dev);
+int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave);
+int wk_dev_sync(wk_dev* dev);
+/* Load every segment that fits the budget (warm cache). */
+int wk_dev_load_all(wk_dev* dev);
+/* GPUCache::load_segment of (index, pid, dir). */
+int wk_seg_ensure(wk_dev* dev, uint32_t index, uint32_t pid, int32_t dir);
+int wk_seg_resident(wk_dev* dev, uint32_t index, uint32_t pid, int32_t dir, int32_t* resident);
+
+/* ----------------------------------------------------------------- tables */
+
+int wk_table_upload(wk_dev* dev, const uint32_t* host, uint64_t rows, uint32_t cols, wk_table** out);
+int wk_table_shape(const wk_table* t, uint64_t* rows, uint32_t* cols);
+int wk_table_download(wk_dev* dev, const wk_table* t, uint32_t* dst, uint64_t cap_elems);
+int wk_table_free(wk_dev* dev, wk_table* t);
+
+/* --------------------------------------------------------------- patterns */
+
+/* index_to_unknown: ?X from the index list [0|tpid|dir] (1 column). */
+int wk_start_i2u(wk_dev* dev, uint32_t tpid, int32_t dir, wk_table** out);
+/* const_to_unknown: ?X from [vid|pid|dir] (1 column). */
+int wk_start_c2u(wk_dev* dev, uint32_t vid, uint32_t pid, int32_t dir, wk_table** out);
+/* known_to_unknown: append column of neighbours of row[start_col]. */
+int wk_expand_k2u(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir,
+                  wk_table** out);
+/* known_to_known: keep rows with row[end_col] in neighbours of row[start_col]. */
+int wk_expand_k2k(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir,
+                  uint32_t end_col, wk_table** out);
+/* known_to_const: keep rows with `end` in neighbours of row[start_col]. */
+int wk_expand_k2c(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir,
+                  uint32_t end, wk_table** out);
+
+/* ------------------------------------------------------------------ query */
+
+/* Run one query pattern by pattern (per-query GPU execution, one host sync
+ * per pattern as in the reference).  On success the history table stays on
+ * the device as the handle's last result: shape and var->column map are
+ * returned here, rows are fetched with wk_result_download. */
+int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols, int32_t* v2c);
+int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems);
+
+/* ------------------------------------------------------------------ batch */
+
+typedef struct {
+  uint64_t* rows;     /* [nq] result rows */
+  uint32_t* cols;     /* [nq] result columns */
+  int32_t* status;    /* [nq] per-query status code */
+  uint64_t* offset;   /* [nq] element offset of the table in the batch arena */
+  int32_t* v2c;       /* [nq * WK_MAX_VARS] var->col maps (-1 = unbound) */
+} wk_batch_result;
+
+/* Run nq queries together: each pattern step of all queries is one set of
+ * launches over a concatenated table with per-query offsets, captured in a
+ * hipGraph when opts allow.  Result tables stay in the batch arena. */
+int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph,
+                 wk_batch_result* out);
+int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems);
+
+/* ---------------------------------------------------------------- serving */
+
+typedef struct {
+  uint32_t max_inflight;     /* queries admitted at once */
+  uint64_t footprint_budget; /* bytes of projected history tables admitted */
+  int32_t per_query;         /* 1: FIFO, one query at a time (per-query baseline) */
+  int32_t keep_results;      /* 1: keep each result's rows (for checking) */
+} wk_serve_opts;
+
+typedef struct {
+  double* latency_us;        /* [nq] completion - arrival */
+  double* start_us;          /* [nq] admission - arrival */
+  uint64_t* rows;            /* [nq] */
+  int32_t* status;           /* [nq] */
+  uint64_t* checksum;        /* [nq] order-independent hash of the rows */
+  double wall_s;             /* first arrival -> last completion */
+  uint64_t batches;          /* scheduler iterations that launched work */
+} wk_serve_result;
+
+/* Open-loop replay: query i arrives at arrival_s[i] seconds after the start
+ * (non-decreasing).  A native scheduler admits arrived queries as slots and
+ * footprint free up and advances every in-flight query by one pattern step
+ * per iteration (continuous batching). */
+int wk_serve_replay(wk_dev* dev, const wk_query* qs, const double* arrival_s, uint32_t nq,
+                    const wk_serve_opts* opts, wk_serve_result* out);
+/* Rows of query i kept by the last replay with keep_results = 1. */
+int wk_serve_result_rows(wk_dev* dev, uint32_t i, uint32_t* dst, uint64_t cap_elems,
+                         uint64_t* rows, uint32_t* cols);
+
+/* Order-independent checksum of a host table (same function the serving
+ * path uses): sum over rows of hash_u64 of a per-row fold. */
+uint64_t wk_table_checksum(const uint32_t* table, uint64_t rows, uint32_t cols);
+
+/* ------------------------------------------------------------ generators */
+
+/* Seeded university-schema generator: flat (s,p,o) triples, caller frees
+ * with wk_free.  Ids follow the datagen conventions (datagen/generate_data.cpp:357-373):
+ * predicates/types from 2, vertices from 2^17. */
+int wk_gen_university(uint32_t seed, uint32_t n_univ, uint32_t** triples, uint64_t* n);
+/* Id of a schema name ("rdf:type", "ub:memberOf", "ub:GraduateStudent"...); 0 if unknown. */
+uint32_t wk_univ_id(const char* name);
+/* Seeded power-law generator (Zipf predicates, Pareto out-degree). */
+int wk_gen_powerlaw(uint32_t seed, uint64_t n_vertices, uint64_t n_triples, uint32_t n_preds,
+                    uint32_t n_types, double zipf_s, double pareto_alpha, uint64_t max_degree,
+                    uint32_t** triples, uint64_t* n);
+void wk_free(void* p);
+
+/* ------------------------------------------------------------------ misc */
+
+const char* wk_strerror(int code);
+int wk_last_hip_error(void);
+int wk_device_count(int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WK_GPU_H */

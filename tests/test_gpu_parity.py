@@ -1,0 +1,174 @@
+"""Parity of the HIP query path with the CPU oracle (bit-exact multisets of rows).
+
+Every call goes through the C ABI (libwukong_gpu.so); the oracle is only the
+checker."""
+import numpy as np
+import pytest
+
+import oracle
+import wukong_gpu as wg
+from wukong_gpu import queries as Q
+
+from _graphs import random_graph, random_query, same_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def check_query(dev, orc, pats, nvars):
+    want, wv2c, wrc = orc.query(pats, nvars)
+    got, gv2c, grc = dev.query(pats, nvars)
+    assert grc == wrc, (pats, grc, wrc)
+    if wrc == 0:
+        assert gv2c == wv2c, (pats, gv2c, wv2c)
+        assert got.shape == want.shape, (pats, got.shape, want.shape)
+        assert same_rows(got, want), pats
+    return want
+
+
+def test_config1_chain(univ10_dev, univ10_oracle):
+    pats, nv = Q.chain3()
+    t = check_query(univ10_dev, univ10_oracle, pats, nv)
+    assert t.shape[0] > 20000 and t.shape[1] == 3
+
+
+@pytest.mark.parametrize("name", ["lubm_q7", "type_start2", "lubm_q1"])
+def test_lubm_shapes(univ10_dev, univ10_oracle, name):
+    pats, nv = getattr(Q, name)()
+    t = check_query(univ10_dev, univ10_oracle, pats, nv)
+    assert t.shape[0] > 0
+
+
+def test_per_pattern_api(univ10_dev, univ10_oracle):
+    """i2u -> k2u -> k2c -> k2k through the per-pattern entry points."""
+    dev, orc = univ10_dev, univ10_oracle
+    GS, MEM, SUB, ADV, WF = (Q.U("ub:GraduateStudent"), Q.U("ub:memberOf"), Q.U("ub:subOrganizationOf"),
+                             Q.U("ub:advisor"), Q.U("ub:worksFor"))
+    t0 = dev.i2u(GS, wg.IN)
+    a = t0.numpy()
+    assert same_rows(a, orc.get_index(GS, wg.IN).reshape(-1, 1))
+    t1 = dev.k2u(t0, 0, ADV, wg.OUT)          # ?x advisor ?p
+    t2 = dev.k2u(t1, 1, WF, wg.OUT)           # ?p worksFor ?d
+    t3 = dev.k2k(t2, 0, MEM, wg.OUT, 2)       # ?x memberOf ?d (known_to_known)
+    dept = int(t3.numpy()[0, 2])
+    t4 = dev.k2c(t3, 2, SUB, wg.OUT, int(orc.get_triples(dept, SUB, wg.OUT)[0]))
+    raw = [(GS, wg.TYPE_ID, -1, wg.IN), (-1, ADV, -2, wg.OUT), (-2, WF, -3, wg.OUT), (-1, MEM, -3, wg.OUT),
+           (-3, SUB, int(orc.get_triples(dept, SUB, wg.OUT)[0]), wg.OUT)]
+    for k, t in enumerate([t1, t2, t3, t4]):
+        want, _, rc = orc.query(raw[:k + 2], 3)
+        assert rc == 0
+        assert same_rows(t.numpy(), want), k
+    # a table uploaded from the host expands the same way
+    up = dev.upload(a)
+    assert same_rows(dev.k2u(up, 0, ADV, wg.OUT).numpy(), t1.numpy())
+
+
+def test_type_index_expansion(univ10_dev, univ10_oracle):
+    """k2u with rdf:type IN reads the type index (sparql.hpp:342-343)."""
+    v = int(univ10_oracle.get_index(Q.U("ub:FullProfessor"), wg.IN)[0])
+    pats = [(v, wg.TYPE_ID, -1, wg.OUT), (-1, wg.TYPE_ID, -2, wg.IN)]
+    t = check_query(univ10_dev, univ10_oracle, pats, 2)
+    assert t.shape[0] > 100
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_graphs(seed):
+    tri, info = random_graph(seed, tpid_objects=(seed % 2 == 0))
+    orc = oracle.Store(tri)
+    st = wg.Store(tri, load_factor_pct=95 if seed % 3 == 0 else 0)  # 95%: long bucket chains
+    dev = wg.Device(st, table_budget=64 << 20)
+    rng = np.random.default_rng(100 + seed)
+    for _ in range(60):
+        pats, nv = random_query(rng, info)
+        check_query(dev, orc, pats, nv)
+    dev.close()
+
+
+@pytest.mark.parametrize("pw", [1, 2, 4, 8])
+@pytest.mark.parametrize("rpw", [0, 1, 4, 16, 64])
+def test_tuning_knobs(univ10_dev, univ10_oracle, pw, rpw):
+    univ10_dev.set_tuning(pw, rpw)
+    try:
+        for f in (Q.chain3, Q.lubm_q7):
+            pats, nv = f()
+            check_query(univ10_dev, univ10_oracle, pats, nv)
+    finally:
+        univ10_dev.set_tuning(8, 0)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_oracle(univ10_dev, univ10_oracle, univ10_store, use_graph):
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 300, seed=5)
+    qs = [(p, nv) for p, nv, _ in stream]
+    for rep in range(3):  # graph replays with eager work in between
+        res = univ10_dev.batch(qs, use_graph=use_graph)
+        for (pats, nv), (t, v2c, rc) in zip(qs, res):
+            want, wv2c, wrc = univ10_oracle.query(pats, nv)
+            assert rc == wrc
+            assert v2c == wv2c
+            assert same_rows(t, want), pats
+        pats, nv = Q.chain3()
+        check_query(univ10_dev, univ10_oracle, pats, nv)
+
+
+def test_batch_mixed_random(univ10_oracle):
+    tri, info = random_graph(11)
+    orc = oracle.Store(tri)
+    dev = wg.Device(wg.Store(tri), table_budget=64 << 20)
+    rng = np.random.default_rng(7)
+    qs = [random_query(rng, info) for _ in range(200)]
+    res = dev.batch(qs, use_graph=True)
+    for (pats, nv), (t, v2c, rc) in zip(qs, res):
+        want, wv2c, wrc = orc.query(pats, nv)
+        assert rc == wrc, pats
+        if rc == 0:
+            assert v2c == wv2c
+            assert same_rows(t, want), pats
+    dev.close()
+
+
+def test_batch_overflow_splits(univ10_store, univ10_oracle):
+    """A table budget far below the batch's total output forces splitting;
+    a query that alone exceeds it reports WK_E_NOMEM_BUDGET."""
+    dev = wg.Device(univ10_store, table_budget=4 << 20)   # ~150K elements per table
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 200, seed=9)
+    qs = [(p, nv) for p, nv, _ in stream] + [Q.type_start2()]
+    res = dev.batch(qs, use_graph=False)
+    for (pats, nv), (t, v2c, rc) in zip(qs[:-1], res[:-1]):
+        want, _, _ = univ10_oracle.query(pats, nv)
+        assert rc == 0 and same_rows(t, want)
+    assert res[-1][2] == wg.WK_E_NOMEM_BUDGET
+    dev.close()
+
+
+def test_gstore_swapping(univ10_store, univ10_oracle):
+    """A gstore budget below the store size evicts and reloads segments."""
+    info = univ10_store.info()
+    total = info["num_buckets"] * 128 + info["num_values"] * 4
+    # the largest single segment (type OUT keys, index IN values) still fits
+    dev = wg.Device(univ10_store, gstore_budget=total * 2 // 5, key_block_shift=10, value_block_shift=14,
+                    table_budget=256 << 20)
+    for f in (Q.chain3, Q.lubm_q7, Q.type_start2, Q.lubm_q1, Q.chain3):
+        pats, nv = f()
+        check_query(dev, univ10_oracle, pats, nv)
+    s = dev.stats()
+    assert s["segments_evicted"] > 0 and s["h2d_bytes"] > total // 3
+    dev.close()
+
+
+def test_serve_results_alone_and_in_mix(univ10_dev, univ10_oracle, univ10_store):
+    """Continuous batching: each query's rows equal the oracle's (alone) and
+    its checksum equals the per-query path's."""
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 400, seed=3)
+    qs = [(p, nv) for p, nv, _ in stream]
+    arr = np.cumsum(np.random.default_rng(1).exponential(1 / 20000.0, size=len(qs)))
+    r = univ10_dev.serve(qs, arr, max_inflight=64, keep_results=True)
+    assert (r["status"] == 0).all()
+    for i, (pats, nv) in enumerate(qs):
+        want, _, _ = univ10_oracle.query(pats, nv)
+        got = univ10_dev.served_rows(i)
+        assert same_rows(got, want), i
+        assert int(r["checksum"][i]) == (wg.checksum(want) if want.size else 0)
+    pq = univ10_dev.serve(qs, arr, per_query=True)
+    assert np.array_equal(pq["checksum"], r["checksum"])
+    assert np.array_equal(pq["rows"], r["rows"])
+    assert r["batches"] < len(qs)

@@ -1,0 +1,147 @@
+// Batched execution: all queries of a batch advance one pattern step per set
+// of launches over one concatenated history table with per-query offsets.
+// Sizes stay on the device between steps, so the whole multi-step sequence
+// is launched without host round trips and can be replayed from a hipGraph.
+#include <algorithm>
+
+#include "device.hpp"
+
+namespace wk {
+
+namespace {
+
+void launch_batch_steps(wk_dev* dev, uint32_t S, uint32_t nq, bool timed) {
+  for (uint32_t k = 0; k < S; k++) {
+    const uint32_t* tin = dev->tbuf[(k + 1) & 1];
+    uint32_t* tout = dev->tbuf[k & 1];
+    const StepArgs a = make_args(dev, k, nq, tin, tout, dev->tcap, k + 1 < S);
+    hipEvent_t ev = nullptr;
+    if (timed) timed_begin(dev, &ev);
+    launch_probe(a, dev->opts.probe_width, 0, dev->stream);
+    if (timed) timed_end(dev, ev, "batch_probe", 0);
+    if (timed) timed_begin(dev, &ev);
+    launch_scan(a, 0, dev->stream);
+    if (timed) timed_end(dev, ev, "batch_scan", 0);
+    if (timed) timed_begin(dev, &ev);
+    launch_gather(a, dev->opts.rows_per_wave, 0, dev->stream);
+    if (timed) timed_end(dev, ev, "batch_gather", 0);
+  }
+}
+
+}  // namespace
+
+int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+              bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end) {
+  const uint32_t nq = q1 - q0;
+  *arena_end = arena_base;
+  uint32_t S = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    rows[q] = 0;
+    offsets[q] = 0;
+    status[q] = plans[q].status;
+    if (plans[q].status == WK_OK) S = std::max<uint32_t>(S, uint32_t(plans[q].steps.size()));
+  }
+  if (nq == 0 || S == 0) return WK_OK;
+  int rc = reserve_steps(dev, S, nq);
+  if (rc) return rc;
+  std::vector<int> segs;
+  for (uint32_t q = q0; q < q1; q++)
+    if (plans[q].status == WK_OK)
+      for (const auto& sd : plans[q].steps)
+        if (sd.seg >= 0 && std::find(segs.begin(), segs.end(), sd.seg) == segs.end()) segs.push_back(sd.seg);
+  rc = ensure_segments(dev, segs);
+  if (rc) return rc;
+
+  const size_t qc = dev->nq_cap;
+  uint64_t nvalid = 0;
+  for (uint32_t k = 0; k <= S; k++) {
+    for (uint32_t i = 0; i < nq; i++) {
+      const Plan& p = plans[q0 + i];
+      const uint32_t ns = p.status == WK_OK ? uint32_t(p.steps.size()) : 0;
+      StepDesc d{};
+      if (k < ns) {
+        d = p.steps[k];
+        d.final_step = (k + 1 == ns) ? 1 : 0;
+      } else {
+        d.op = OP_SKIP;
+        d.seg = -1;
+      }
+      d.in_rows = (k == 0 && ns > 0) ? 1 : 0;
+      d.in_off = 0;
+      d.out_rows = d.out_off = 0;
+      dev->h_desc[k * qc + i] = d;
+    }
+  }
+  for (uint32_t i = 0; i < nq; i++) {
+    dev->h_rs[i] = nvalid;
+    nvalid += dev->h_desc[i].in_rows;
+  }
+  dev->h_rs[nq] = nvalid;
+  for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
+  dev->h_state[0].arena_base = arena_base;
+  auto fail = [&](int code) {
+    unpin_segments(dev, segs);
+    return code;
+  };
+  if (hipMemcpyAsync(dev->d_desc, dev->h_desc, (S + 1) * qc * sizeof(StepDesc), hipMemcpyHostToDevice,
+                     dev->stream) ||
+      hipMemcpyAsync(dev->d_rs, dev->h_rs, (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream) ||
+      hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState), hipMemcpyHostToDevice,
+                     dev->stream))
+    return fail(WK_E_HIP);
+
+  if (use_graph) {
+    auto key = std::make_pair(S, nq);
+    auto it = dev->graphs.find(key);
+    if (it == dev->graphs.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      if (hipStreamBeginCapture(dev->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return fail(WK_E_HIP);
+      launch_batch_steps(dev, S, nq, false);
+      if (hipStreamEndCapture(dev->stream, &g) != hipSuccess) return fail(WK_E_HIP);
+      if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(WK_E_HIP);
+      }
+      (void)hipGraphDestroy(g);
+      it = dev->graphs.emplace(key, ge).first;
+    }
+    if (hipGraphLaunch(it->second, dev->stream) != hipSuccess) return fail(WK_E_HIP);
+  } else {
+    launch_batch_steps(dev, S, nq, dev->opts.profile != 0);
+  }
+  if (hipMemcpyAsync(dev->h_desc, dev->d_desc, S * qc * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream) ||
+      hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream) ||
+      hipStreamSynchronize(dev->stream))
+    return fail(WK_E_HIP);
+  unpin_segments(dev, segs);
+  bool overflow = false;
+  for (uint32_t k = 0; k < S; k++) overflow |= dev->h_state[k].status != 0;
+  if (overflow) {
+    if (nq == 1) {
+      status[q0] = WK_E_NOMEM_BUDGET;
+      return WK_OK;
+    }
+    // split the batch; halves reuse the tables and append to the arena
+    const uint32_t mid = q0 + nq / 2;
+    uint64_t end1 = arena_base;
+    rc = run_batch(dev, plans, q0, mid, arena_base, use_graph, rows, offsets, status, &end1);
+    if (rc) return rc;
+    return run_batch(dev, plans, mid, q1, end1, use_graph, rows, offsets, status, arena_end);
+  }
+  uint64_t end = arena_base;
+  for (uint32_t i = 0; i < nq; i++) {
+    const uint32_t q = q0 + i;
+    const Plan& p = plans[q];
+    if (p.status != WK_OK || p.steps.empty()) continue;
+    const StepDesc& d = dev->h_desc[(p.steps.size() - 1) * qc + i];
+    rows[q] = d.out_rows;
+    offsets[q] = d.out_off;
+    end = std::max<uint64_t>(end, d.out_off + d.out_rows * p.cols);
+  }
+  *arena_end = end;
+  dev->stats.queries_run += nq;
+  return WK_OK;
+}
+
+}  // namespace wk

@@ -1,0 +1,599 @@
+// extern "C" entry points of include/wk_gpu.h (store, device, tables,
+// per-pattern calls, per-query and batched execution).
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "device.hpp"
+
+using namespace wk;
+
+struct wk_store {
+  Store* s;
+};
+
+namespace {
+
+uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+void free_dev(wk_dev* d) {
+  if (!d) return;
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  if (d->copy_stream) (void)hipStreamSynchronize(d->copy_stream);
+  for (auto& g : d->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& p : d->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : d->event_pool) (void)hipEventDestroy(e);
+  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs, d->d_ktab, d->d_vtab, d->tbuf[0], d->tbuf[1], d->arena,
+                      d->cnt,    d->eoff,   d->pfx,    d->tile_sum, d->tile_row, d->d_desc, d->d_rs, d->d_state};
+  for (void* p : dev_ptrs)
+    if (p) (void)hipFree(p);
+  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state};
+  for (void* p : host_ptrs)
+    if (p) (void)hipHostFree(p);
+  if (d->copy_done) (void)hipEventDestroy(d->copy_done);
+  if (d->compute_done) (void)hipEventDestroy(d->compute_done);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
+  if (d->copy_stream) (void)hipStreamDestroy(d->copy_stream);
+  delete d;
+}
+
+int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
+  d->st = st;
+  d->opts = *o;
+  if (d->opts.probe_width != 1 && d->opts.probe_width != 2 && d->opts.probe_width != 4) d->opts.probe_width = 8;
+  if (d->opts.rows_per_wave != 1 && d->opts.rows_per_wave != 4 && d->opts.rows_per_wave != 16 &&
+      d->opts.rows_per_wave != 64)
+    d->opts.rows_per_wave = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return WK_E_NO_DEVICE;
+  }
+  if (o->device < 0 || o->device >= n) return WK_E_INVALID_ARG;
+  d->device = o->device;
+  WK_HIP(hipSetDevice(d->device));
+  WK_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  WK_HIP(hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
+  WK_HIP(hipEventCreateWithFlags(&d->copy_done, hipEventDisableTiming));
+  WK_HIP(hipEventCreateWithFlags(&d->compute_done, hipEventDisableTiming));
+  d->kshift = o->key_block_shift ? o->key_block_shift : 14;
+  d->vshift = o->value_block_shift ? o->value_block_shift : 19;
+  if (d->kshift < 4 || d->kshift > 24 || d->vshift < 8 || d->vshift > 28) return WK_E_INVALID_ARG;
+
+  // segment -> block tables
+  const size_t ns = st->segs.size();
+  d->seg.resize(ns);
+  WK_HIP(hipHostMalloc(&d->h_segs, std::max<size_t>(ns, 1) * sizeof(DevSeg), hipHostMallocDefault));
+  uint64_t kt = 0, vt = 0;
+  for (size_t i = 0; i < ns; i++) {
+    const SegMeta& m = st->segs[i];
+    if (m.num_edges >= (uint64_t(1) << 32)) return WK_E_OVERFLOW;
+    SegSlot& s = d->seg[i];
+    s.nkb = uint32_t(ceil_div(m.total_buckets(), uint64_t(1) << d->kshift));
+    s.nvb = uint32_t(ceil_div(m.num_edges, uint64_t(1) << d->vshift));
+    DevSeg& h = d->h_segs[i];
+    h = DevSeg{};
+    h.num_buckets = m.num_buckets;
+    h.total_buckets = m.total_buckets();
+    h.edge_start = m.edge_start;
+    h.num_edges = m.num_edges;
+    h.kb_base = uint32_t(kt);
+    h.vb_base = uint32_t(vt);
+    h.resident = 0;
+    kt += s.nkb;
+    vt += s.nvb;
+  }
+  d->ktab_n = kt;
+  d->vtab_n = vt;
+  WK_HIP(hipHostMalloc(&d->h_ktab, std::max<uint64_t>(kt, 1) * sizeof(uint32_t), hipHostMallocDefault));
+  WK_HIP(hipHostMalloc(&d->h_vtab, std::max<uint64_t>(vt, 1) * sizeof(uint32_t), hipHostMallocDefault));
+  std::memset(d->h_ktab, 0, std::max<uint64_t>(kt, 1) * sizeof(uint32_t));
+  std::memset(d->h_vtab, 0, std::max<uint64_t>(vt, 1) * sizeof(uint32_t));
+  WK_HIP(hipMalloc(&d->d_segs, std::max<size_t>(ns, 1) * sizeof(DevSeg)));
+  WK_HIP(hipMalloc(&d->d_ktab, std::max<uint64_t>(kt, 1) * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->d_vtab, std::max<uint64_t>(vt, 1) * sizeof(uint32_t)));
+  WK_HIP(hipMemcpy(d->d_segs, d->h_segs, std::max<size_t>(ns, 1) * sizeof(DevSeg), hipMemcpyHostToDevice));
+  WK_HIP(hipMemcpy(d->d_ktab, d->h_ktab, std::max<uint64_t>(kt, 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  WK_HIP(hipMemcpy(d->d_vtab, d->h_vtab, std::max<uint64_t>(vt, 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+
+  // gstore cache: enough blocks for the whole store, or the budget split in
+  // proportion to key and value bytes (GPUCache ctor, gpu_cache.hpp:428-470)
+  const uint64_t kbytes = (uint64_t(1) << d->kshift) * ASSOCIATIVITY * sizeof(Slot);
+  const uint64_t vbytes = (uint64_t(1) << d->vshift) * sizeof(uint32_t);
+  const uint64_t need = kt * kbytes + vt * vbytes;
+  uint64_t nk = kt, nv = vt;
+  if (o->gstore_budget && o->gstore_budget < need) {
+    const double kshare = double(kt * kbytes) / double(std::max<uint64_t>(need, 1));
+    nk = uint64_t(double(o->gstore_budget) * kshare) / kbytes;
+    nv = (o->gstore_budget - nk * kbytes) / vbytes;
+  }
+  nk = std::max<uint64_t>(nk, 1);
+  nv = std::max<uint64_t>(nv, 1);
+  d->nkblk = uint32_t(nk);
+  d->nvblk = uint32_t(nv);
+  WK_HIP(hipMalloc(&d->karena, nk * kbytes));
+  WK_HIP(hipMalloc(&d->varena, nv * vbytes));
+  for (uint64_t i = nk; i-- > 0;) d->free_k.push_back(uint32_t(i));
+  for (uint64_t i = nv; i-- > 0;) d->free_v.push_back(uint32_t(i));
+
+  // history tables + scratch: 2 ping-pong tables + arena (4 B each per
+  // element) and per-row count/offset/prefix (16 B), rows <= elements
+  const uint64_t tb = o->table_budget ? o->table_budget : (uint64_t(8) << 30);
+  d->tcap = std::max<uint64_t>(tb / 28, 1 << 16);
+  d->arena_cap = d->tcap;
+  d->rcap = d->tcap;
+  d->tile_cap = d->rcap / kTile + 2;
+  d->otile_cap = (d->tcap + d->arena_cap) / kOutTile + 2;
+  WK_HIP(hipMalloc(&d->tbuf[0], d->tcap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->tbuf[1], d->tcap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->arena, d->arena_cap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->cnt, d->rcap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->eoff, d->rcap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->pfx, (d->rcap + 1) * sizeof(uint64_t)));
+  WK_HIP(hipMalloc(&d->tile_sum, d->tile_cap * sizeof(uint64_t)));
+  WK_HIP(hipMalloc(&d->tile_row, (d->otile_cap + 1) * sizeof(uint64_t)));
+  int rc = reserve_steps(d, 8, 64);
+  if (rc) return rc;
+  d->stats.key_blocks_total = nk;
+  d->stats.value_blocks_total = nv;
+  d->stats.table_capacity_elems = d->tcap;
+  return WK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ store
+
+int wk_store_build(const uint32_t* pso, uint64_t n_pso, const uint32_t* pos, uint64_t n_pos,
+                   const wk_store_opts* opts, wk_store** out) {
+  if (!out) return WK_E_INVALID_ARG;
+  *out = nullptr;
+  BuildOpts bo;
+  if (opts) {
+    bo.pinned = opts->pinned != 0;
+    if (opts->load_factor_pct > 0) bo.load_factor_pct = opts->load_factor_pct;
+  }
+  Store* s = nullptr;
+  int rc;
+  try {
+    rc = build_store(pso, n_pso, pos, n_pos, bo, &s);
+  } catch (const std::bad_alloc&) {
+    return WK_E_NOMEM_HOST;
+  }
+  if (rc) return rc;
+  *out = new (std::nothrow) wk_store{s};
+  if (!*out) {
+    delete s;
+    return WK_E_NOMEM_HOST;
+  }
+  return WK_OK;
+}
+
+void wk_store_destroy(wk_store* st) {
+  if (!st) return;
+  delete st->s;
+  delete st;
+}
+
+int wk_store_info_get(const wk_store* st, wk_store_info* info) {
+  if (!st || !info) return WK_E_INVALID_ARG;
+  info->num_triples_out = st->s->n_out;
+  info->num_triples_in = st->s->n_in;
+  info->num_keys = st->s->total_keys;
+  info->num_buckets = st->s->total_buckets;
+  info->num_values = st->s->total_values;
+  info->num_segments = uint32_t(st->s->segs.size());
+  info->pinned = st->s->pinned ? 1 : 0;
+  return WK_OK;
+}
+
+int wk_store_segment(const wk_store* st, uint32_t i, wk_seg_info* seg) {
+  if (!st || !seg || i >= st->s->segs.size()) return WK_E_INVALID_ARG;
+  const SegMeta& m = st->s->segs[i];
+  seg->index = m.id.index;
+  seg->pid = m.id.pid;
+  seg->dir = m.id.dir;
+  seg->pad = 0;
+  seg->num_keys = m.num_keys;
+  seg->num_buckets = m.num_buckets;
+  seg->num_ext_buckets = m.num_ext;
+  seg->bucket_start = m.bucket_start;
+  seg->num_edges = m.num_edges;
+  seg->edge_start = m.edge_start;
+  return WK_OK;
+}
+
+int wk_store_raw(const wk_store* st, const void** slots, uint64_t* num_buckets, const uint32_t** values,
+                 uint64_t* num_values) {
+  if (!st) return WK_E_INVALID_ARG;
+  if (slots) *slots = st->s->slots;
+  if (num_buckets) *num_buckets = st->s->total_buckets;
+  if (values) *values = st->s->values;
+  if (num_values) *num_values = st->s->total_values;
+  return WK_OK;
+}
+
+int wk_store_get_triples(const wk_store* st, uint32_t vid, uint32_t pid, int32_t dir, const uint32_t** edges,
+                         uint64_t* size) {
+  if (!st || !edges || !size) return WK_E_INVALID_ARG;
+  if (dir != WK_IN && dir != WK_OUT) return WK_E_INVALID_ARG;
+  *edges = st->s->get_triples(vid, pid, dir, *size);
+  return WK_OK;
+}
+
+int wk_store_get_index(const wk_store* st, uint32_t tpid, int32_t dir, const uint32_t** edges, uint64_t* size) {
+  if (!st || !edges || !size) return WK_E_INVALID_ARG;
+  if (dir != WK_IN && dir != WK_OUT) return WK_E_INVALID_ARG;
+  *edges = st->s->get_index(tpid, dir, *size);
+  return WK_OK;
+}
+
+// ----------------------------------------------------------------- device
+
+int wk_dev_create(const wk_store* st, const wk_dev_opts* opts, wk_dev** out) {
+  if (!st || !out) return WK_E_INVALID_ARG;
+  *out = nullptr;
+  wk_dev_opts o{};
+  if (opts) o = *opts;
+  wk_dev* d = new (std::nothrow) wk_dev();
+  if (!d) return WK_E_NOMEM_HOST;
+  int rc;
+  try {
+    rc = create_dev(st->s, &o, d);
+  } catch (const std::bad_alloc&) {
+    rc = WK_E_NOMEM_HOST;
+  }
+  if (rc) {
+    free_dev(d);
+    return rc;
+  }
+  *out = d;
+  return WK_OK;
+}
+
+void wk_dev_destroy(wk_dev* dev) { free_dev(dev); }
+
+int wk_dev_stats_get(wk_dev* dev, wk_dev_stats* s) {
+  if (!dev || !s) return WK_E_INVALID_ARG;
+  *s = dev->stats;
+  s->key_blocks_free = dev->free_k.size();
+  s->value_blocks_free = dev->free_v.size();
+  return WK_OK;
+}
+
+int wk_dev_kernel_stats(wk_dev* dev, wk_kernel_stat* out, uint32_t cap, uint32_t* n) {
+  if (!dev || !n) return WK_E_INVALID_ARG;
+  int rc = flush_timings(dev);
+  if (rc) return rc;
+  uint32_t i = 0;
+  for (auto& e : dev->kstats) {
+    if (out && i < cap) {
+      std::memset(&out[i], 0, sizeof(wk_kernel_stat));
+      std::strncpy(out[i].name, e.first.c_str(), sizeof(out[i].name) - 1);
+      out[i].launches = e.second.launches;
+      out[i].total_ms = e.second.ms;
+      out[i].alg_bytes = e.second.bytes;
+    }
+    i++;
+  }
+  *n = i;
+  return WK_OK;
+}
+
+int wk_dev_reset_kernel_stats(wk_dev* dev) {
+  if (!dev) return WK_E_INVALID_ARG;
+  int rc = flush_timings(dev);
+  dev->kstats.clear();
+  return rc;
+}
+
+int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
+  if (!dev) return WK_E_INVALID_ARG;
+  if (probe_width != 1 && probe_width != 2 && probe_width != 4 && probe_width != 8) return WK_E_INVALID_ARG;
+  if (rows_per_wave != 0 && rows_per_wave != 1 && rows_per_wave != 4 && rows_per_wave != 16 &&
+      rows_per_wave != 64)
+    return WK_E_INVALID_ARG;
+  if (dev->opts.probe_width != probe_width || dev->opts.rows_per_wave != rows_per_wave) {
+    (void)hipStreamSynchronize(dev->stream);
+    for (auto& g : dev->graphs) (void)hipGraphExecDestroy(g.second);
+    dev->graphs.clear();
+  }
+  dev->opts.probe_width = probe_width;
+  dev->opts.rows_per_wave = rows_per_wave;
+  return WK_OK;
+}
+
+int wk_dev_sync(wk_dev* dev) {
+  if (!dev) return WK_E_INVALID_ARG;
+  WK_HIP(hipStreamSynchronize(dev->copy_stream));
+  WK_HIP(hipStreamSynchronize(dev->stream));
+  return WK_OK;
+}
+
+int wk_dev_load_all(wk_dev* dev) {
+  if (!dev) return WK_E_INVALID_ARG;
+  std::vector<int> none;
+  for (size_t i = 0; i < dev->seg.size(); i++) {
+    std::vector<int> one{int(i)};
+    int rc = ensure_segments(dev, one);
+    unpin_segments(dev, one);
+    if (rc == WK_E_NOMEM_BUDGET || rc == WK_E_SEG_TOO_LARGE) break;
+    if (rc) return rc;
+  }
+  return wk_dev_sync(dev);
+}
+
+int wk_seg_ensure(wk_dev* dev, uint32_t index, uint32_t pid, int32_t dir) {
+  if (!dev || (dir != WK_IN && dir != WK_OUT)) return WK_E_INVALID_ARG;
+  const int s = dev->st->seg_index(index, pid, uint32_t(dir));
+  if (s < 0) return WK_E_INVALID_ARG;
+  std::vector<int> one{s};
+  int rc = ensure_segments(dev, one);
+  unpin_segments(dev, one);
+  return rc;
+}
+
+int wk_seg_resident(wk_dev* dev, uint32_t index, uint32_t pid, int32_t dir, int32_t* resident) {
+  if (!dev || !resident) return WK_E_INVALID_ARG;
+  const int s = dev->st->seg_index(index, pid, uint32_t(dir));
+  *resident = (s >= 0 && dev->seg[size_t(s)].resident) ? 1 : 0;
+  return WK_OK;
+}
+
+// ----------------------------------------------------------------- tables
+
+int wk_table_upload(wk_dev* dev, const uint32_t* host, uint64_t rows, uint32_t cols, wk_table** out) {
+  if (!dev || !out || (rows * cols && !host)) return WK_E_INVALID_ARG;
+  *out = nullptr;
+  wk_table* t = new (std::nothrow) wk_table();
+  if (!t) return WK_E_NOMEM_HOST;
+  t->rows = rows;
+  t->cols = cols;
+  if (rows * cols) {
+    if (hipMalloc(&t->d, rows * cols * sizeof(uint32_t)) != hipSuccess) {
+      delete t;
+      return WK_E_NOMEM_BUDGET;
+    }
+    if (hipMemcpy(t->d, host, rows * cols * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(t->d);
+      delete t;
+      return WK_E_HIP;
+    }
+  }
+  *out = t;
+  return WK_OK;
+}
+
+int wk_table_shape(const wk_table* t, uint64_t* rows, uint32_t* cols) {
+  if (!t) return WK_E_INVALID_ARG;
+  if (rows) *rows = t->rows;
+  if (cols) *cols = t->cols;
+  return WK_OK;
+}
+
+int wk_table_download(wk_dev* dev, const wk_table* t, uint32_t* dst, uint64_t cap_elems) {
+  if (!dev || !t) return WK_E_INVALID_ARG;
+  const uint64_t n = t->rows * t->cols;
+  if (n > cap_elems || (n && !dst)) return WK_E_INVALID_ARG;
+  WK_HIP(hipStreamSynchronize(dev->stream));
+  if (n) WK_HIP(hipMemcpy(dst, t->d, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return WK_OK;
+}
+
+int wk_table_free(wk_dev* dev, wk_table* t) {
+  if (!t) return WK_OK;
+  if (dev) (void)hipStreamSynchronize(dev->stream);
+  if (t->d) (void)hipFree(t->d);
+  delete t;
+  return WK_OK;
+}
+
+// --------------------------------------------------------------- patterns
+
+static int pattern_step(wk_dev* dev, StepDesc sd, const wk_table* in, wk_table** out) {
+  if (!dev || !out) return WK_E_INVALID_ARG;
+  *out = nullptr;
+  wk_table* t = new (std::nothrow) wk_table();
+  if (!t) return WK_E_NOMEM_HOST;
+  t->cols = sd.out_cols;
+  uint64_t rows = 0;
+  int rc = run_single_step(dev, sd, in ? in->d : nullptr, &t->d, 0, true, &rows);
+  if (rc) {
+    delete t;
+    return rc;
+  }
+  t->rows = rows;
+  *out = t;
+  return WK_OK;
+}
+
+int wk_start_i2u(wk_dev* dev, uint32_t tpid, int32_t dir, wk_table** out) {
+  if (!dev || (dir != WK_IN && dir != WK_OUT) || tpid >= (1u << NBITS_IDX)) return WK_E_INVALID_ARG;
+  StepDesc sd{};
+  sd.op = OP_EXPAND;
+  sd.key_mode = KEY_CONST;
+  sd.kstart = 0;
+  sd.pid = tpid;
+  sd.dir = uint32_t(dir);
+  sd.seg = dev->st->seg_index(1, PREDICATE_ID, uint32_t(dir));
+  sd.in_rows = 1;
+  sd.out_cols = 1;
+  return pattern_step(dev, sd, nullptr, out);
+}
+
+int wk_start_c2u(wk_dev* dev, uint32_t vid, uint32_t pid, int32_t dir, wk_table** out) {
+  if (!dev || (dir != WK_IN && dir != WK_OUT) || pid >= (1u << NBITS_IDX)) return WK_E_INVALID_ARG;
+  StepDesc sd{};
+  sd.op = OP_EXPAND;
+  sd.key_mode = KEY_CONST;
+  sd.kstart = vid;
+  sd.pid = pid;
+  sd.dir = uint32_t(dir);
+  const SegId s = segid_of_key(make_key(vid, pid, uint64_t(dir)));
+  sd.seg = dev->st->seg_index(s.index, s.pid, s.dir);
+  sd.in_rows = 1;
+  sd.out_cols = 1;
+  return pattern_step(dev, sd, nullptr, out);
+}
+
+static int known_step(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir,
+                      uint32_t op, uint32_t end, wk_table** out) {
+  if (!dev || !in || !out || (dir != WK_IN && dir != WK_OUT) || pid >= (1u << NBITS_IDX)) return WK_E_INVALID_ARG;
+  if (start_col >= in->cols || (op == OP_FILTER_KNOWN && end >= in->cols)) return WK_E_INVALID_ARG;
+  StepDesc sd{};
+  sd.op = op;
+  sd.pid = pid;
+  sd.dir = uint32_t(dir);
+  sd.sc = start_col;
+  sd.in_cols = in->cols;
+  sd.in_rows = in->rows;
+  if (op == OP_EXPAND && pid == TYPE_ID && dir == WK_IN) {
+    sd.key_mode = KEY_TYPE_INDEX;
+    sd.seg = dev->st->seg_index(1, PREDICATE_ID, DIR_IN);
+  } else {
+    sd.key_mode = KEY_FROM_ROW;
+    sd.seg = dev->st->seg_index(0, pid, uint32_t(dir));
+  }
+  if (op == OP_FILTER_KNOWN) sd.ec = end;
+  if (op == OP_FILTER_CONST) sd.kend = end;
+  sd.out_cols = op == OP_EXPAND ? in->cols + 1 : in->cols;
+  return pattern_step(dev, sd, in, out);
+}
+
+int wk_expand_k2u(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir, wk_table** out) {
+  return known_step(dev, in, start_col, pid, dir, OP_EXPAND, 0, out);
+}
+
+int wk_expand_k2k(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir,
+                  uint32_t end_col, wk_table** out) {
+  return known_step(dev, in, start_col, pid, dir, OP_FILTER_KNOWN, end_col, out);
+}
+
+int wk_expand_k2c(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir, uint32_t end,
+                  wk_table** out) {
+  return known_step(dev, in, start_col, pid, dir, OP_FILTER_CONST, end, out);
+}
+
+// ------------------------------------------------------------------ query
+
+int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols, int32_t* v2c) {
+  if (!dev || !q) return WK_E_INVALID_ARG;
+  dev->last_buf = -1;
+  dev->last_rows = 0;
+  dev->last_cols = 0;
+  Plan plan;
+  compile_query(*dev->st, *q, plan);
+  if (rows) *rows = 0;
+  if (cols) *cols = 0;
+  if (plan.status) return plan.status;
+  uint64_t r = 0;
+  int buf = -1;
+  int rc = run_query_device(dev, plan, &r, &buf);
+  if (rc) return rc;
+  dev->last_buf = buf;
+  dev->last_rows = buf >= 0 ? r : 0;
+  dev->last_cols = plan.cols;
+  if (rows) *rows = dev->last_rows;
+  if (cols) *cols = plan.cols;
+  if (v2c)
+    for (int32_t i = 0; i < q->nvars; i++) v2c[i] = plan.v2c[size_t(i)];
+  return WK_OK;
+}
+
+int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems) {
+  if (!dev) return WK_E_INVALID_ARG;
+  const uint64_t n = dev->last_rows * dev->last_cols;
+  if (n > cap_elems || (n && !dst)) return WK_E_INVALID_ARG;
+  if (n == 0) return WK_OK;
+  if (dev->last_buf < 0) return WK_E_STATE;
+  WK_HIP(hipStreamSynchronize(dev->stream));
+  WK_HIP(hipMemcpy(dst, dev->tbuf[dev->last_buf], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return WK_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out) {
+  if (!dev || (!qs && nq) || !out || !out->rows || !out->cols || !out->status || !out->offset)
+    return WK_E_INVALID_ARG;
+  std::vector<Plan> plans(nq);
+  for (uint32_t i = 0; i < nq; i++) {
+    compile_query(*dev->st, qs[i], plans[i]);
+    out->cols[i] = plans[i].cols;
+    if (out->v2c)
+      for (int32_t v = 0; v < WK_MAX_VARS; v++)
+        out->v2c[size_t(i) * WK_MAX_VARS + v] =
+            (plans[i].status == WK_OK && v < qs[i].nvars) ? plans[i].v2c[size_t(v)] : -1;
+  }
+  uint64_t end = 0;
+  int rc = run_batch(dev, plans, 0, nq, 0, use_graph != 0, out->rows, out->offset, out->status, &end);
+  dev->batch_used = rc ? 0 : end;
+  return rc;
+}
+
+int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {
+  if (!dev) return WK_E_INVALID_ARG;
+  if (used_elems) *used_elems = dev->batch_used;
+  if (!dst) return WK_OK;
+  if (cap_elems < dev->batch_used) return WK_E_INVALID_ARG;
+  if (dev->batch_used)
+    WK_HIP(hipMemcpy(dst, dev->arena, dev->batch_used * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return WK_OK;
+}
+
+// ------------------------------------------------------------------- misc
+
+uint64_t wk_table_checksum(const uint32_t* table, uint64_t rows, uint32_t cols) {
+  uint64_t s = 0;
+  for (uint64_t r = 0; r < rows; r++) s += row_hash(table + r * cols, cols);
+  return s;
+}
+
+const char* wk_strerror(int code) {
+  switch (code) {
+    case WK_OK: return "ok";
+    case WK_E_UNKNOWN: return "something wrong happened";
+    case WK_E_SYNTAX: return "query syntax error";
+    case WK_E_UNKNOWN_PATTERN: return "unsupported triple pattern";
+    case WK_E_ATTR_DISABLE: return "attribute support is disabled";
+    case WK_E_NO_REQUIRED_VAR: return "no required variables";
+    case WK_E_UNSUPPORT_UNION: return "unsupported UNION on attribute results";
+    case WK_E_OBJ_ERROR: return "object should not be an index";
+    case WK_E_VERTEX_INVALID: return "subject or object is not valid";
+    case WK_E_UNKNOWN_SUB: return "triple pattern should not start from unknown subject";
+    case WK_E_SETTING: return "setting error";
+    case WK_E_FIRST_PATTERN: return "const_X_X or index_X_X must be the first pattern";
+    case WK_E_UNKNOWN_FILTER: return "unsupported filter type";
+    case WK_E_FILE_NOT_FOUND: return "file not found";
+    case WK_E_INVALID_ARG: return "invalid argument";
+    case WK_E_INVALID_TRIPLE: return "triple violates the id conventions";
+    case WK_E_NOMEM_HOST: return "host allocation failed";
+    case WK_E_NOMEM_BUDGET: return "device budget exceeded";
+    case WK_E_HIP: return "HIP runtime error";
+    case WK_E_NO_DEVICE: return "no usable GPU";
+    case WK_E_OVERFLOW: return "count overflows its field";
+    case WK_E_SEG_TOO_LARGE: return "segment exceeds the gstore budget";
+    case WK_E_STATE: return "call out of order";
+    case WK_E_TOO_MANY_VARS: return "too many variables";
+    default: return "unknown status";
+  }
+}
+
+int wk_last_hip_error(void) { return g_last_hip_error; }
+
+int wk_device_count(int* n) {
+  if (!n) return WK_E_INVALID_ARG;
+  *n = 0;
+  if (hipGetDeviceCount(n) != hipSuccess) {
+    (void)hipGetLastError();
+    *n = 0;
+  }
+  return WK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+// Device handle: HBM gstore cache, history tables, scratch and the per-step
+// orchestration shared by per-query, batched and served execution.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "store.hpp"
+#include "wk_gpu.h"
+
+namespace wk {
+
+extern thread_local int g_last_hip_error;
+
+#define WK_HIP(x)                            \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) {                  \
+      ::wk::g_last_hip_error = int(e_);      \
+      return WK_E_HIP;                       \
+    }                                        \
+  } while (0)
+
+// A compiled query: one StepDesc per pattern (static fields only), the final
+// var->column map and column count, or an error status
+// (SPARQLEngine::execute_one_pattern dispatch, core/engine/sparql.hpp:932-1055).
+struct Plan {
+  std::vector<StepDesc> steps;
+  std::vector<int32_t> v2c;
+  uint32_t cols = 0;
+  int status = WK_OK;
+};
+
+int compile_query(const Store& st, const wk_query& q, Plan& plan);
+
+struct SegSlot {
+  uint32_t nkb = 0, nvb = 0;      // blocks needed
+  std::vector<uint32_t> kblk, vblk;
+  bool resident = false;
+  uint64_t last_use = 0;
+  int pins = 0;
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double ms = 0;
+  double bytes = 0;
+};
+
+struct PendingTiming {
+  std::string name;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+}  // namespace wk
+
+struct wk_table {
+  uint32_t* d = nullptr;
+  uint64_t rows = 0;
+  uint32_t cols = 0;
+};
+
+struct wk_dev {
+  const wk::Store* st = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr, copy_stream = nullptr;
+  wk_dev_opts opts{};
+
+  // gstore cache
+  uint32_t kshift = 14, vshift = 19;
+  uint32_t nkblk = 0, nvblk = 0;
+  wk::Slot* karena = nullptr;
+  uint32_t* varena = nullptr;
+  std::vector<uint32_t> free_k, free_v;
+  std::vector<wk::SegSlot> seg;
+  wk::DevSeg* h_segs = nullptr;   // pinned mirror
+  wk::DevSeg* d_segs = nullptr;
+  uint32_t* h_ktab = nullptr;     // pinned mirror
+  uint32_t* h_vtab = nullptr;
+  uint32_t* d_ktab = nullptr;
+  uint32_t* d_vtab = nullptr;
+  uint64_t ktab_n = 0, vtab_n = 0;
+  uint64_t tick = 0;
+  hipEvent_t copy_done = nullptr, compute_done = nullptr;
+
+  // tables and scratch
+  uint32_t* tbuf[2] = {nullptr, nullptr};
+  uint64_t tcap = 0;
+  uint32_t* arena = nullptr;
+  uint64_t arena_cap = 0;
+  uint32_t* cnt = nullptr;
+  uint32_t* eoff = nullptr;
+  uint64_t* pfx = nullptr;
+  uint64_t rcap = 0;
+  uint64_t* tile_sum = nullptr;
+  uint64_t tile_cap = 0;
+  uint64_t* tile_row = nullptr;
+  uint64_t otile_cap = 0;
+
+  // step descriptors: device arrays sized for nsteps_cap x nq_cap
+  wk::StepDesc* d_desc = nullptr;
+  uint64_t* d_rs = nullptr;
+  wk::StepState* d_state = nullptr;
+  wk::StepDesc* h_desc = nullptr;   // pinned staging
+  uint64_t* h_rs = nullptr;
+  wk::StepState* h_state = nullptr;
+  uint32_t nq_cap = 0, steps_cap = 0;
+
+  // hipGraph cache for batches: key (steps, nq)
+  std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs;
+
+  // last per-query result
+  int last_buf = -1;
+  uint64_t last_rows = 0;
+  uint32_t last_cols = 0;
+  // last batch
+  uint64_t batch_used = 0;
+
+  // served results kept for checking
+  std::vector<std::vector<uint32_t>> served_rows;
+  std::vector<uint32_t> served_cols;
+
+  wk_dev_stats stats{};
+  std::map<std::string, wk::KernelStat> kstats;
+  std::vector<wk::PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace wk {
+
+// Make `segs` resident (pinned against eviction while in use by the launch
+// sequence that follows); the compute stream waits for their copies.
+int ensure_segments(wk_dev* dev, const std::vector<int>& segs);
+void unpin_segments(wk_dev* dev, const std::vector<int>& segs);
+int prefetch_segment(wk_dev* dev, int seg, const std::vector<int>& in_use);
+
+// Capacity management for step descriptors.
+int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq);
+
+StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin, uint32_t* tout,
+                   uint64_t tcap, bool has_next);
+void launch_step(wk_dev* dev, const StepArgs& a, uint64_t rows_hint, bool with_gather, uint64_t out_hint,
+                 const char* tag, double alg_bytes);
+
+// Timing helpers (active when opts.profile).
+void timed_begin(wk_dev* dev, hipEvent_t* ev);
+void timed_end(wk_dev* dev, hipEvent_t ev, const char* name, double bytes);
+int flush_timings(wk_dev* dev);
+
+// Run one step of a single query on the host-synchronous path; writes the
+// output into `tout` (capacity `tcap` elements) or allocates it when
+// `alloc_out` is set.  Returns rows in *out_rows.
+int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** tout, uint64_t tcap,
+                    bool alloc_out, uint64_t* out_rows, int prefetch_seg = -1);
+
+// Per-query execution, one host sync per pattern; the result table is left
+// in dev->tbuf[*buf] (rows * plan.cols elements).
+int run_query_device(wk_dev* dev, const Plan& plan, uint64_t* rows, int* buf);
+
+// Batched execution of queries [q0, q1) of `plans`; results land in the
+// arena from `arena_base`.  Fills rows/offsets/status per query.
+int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+              bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end);
+
+}  // namespace wk

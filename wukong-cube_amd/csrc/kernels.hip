@@ -1,0 +1,488 @@
+// CDNA4 (gfx950) kernels of the Wukong GPU query path.
+//
+// Replaces the reference's per-pattern CUDA pipeline (src/gpu/gpu_hash.cu):
+// key generation, slot probe and edge lookup (:96-326) are fused into one
+// probe pass; the thrust inclusive_scan (:589-598) becomes a project-written
+// three-phase wave64/LDS scan with 64-bit offsets; the thread-per-row gathers
+// (:525-587,764-836) become an output-balanced gather (each output row finds
+// its source row by a bounded binary search over the 64-bit prefix) or a
+// row-centric gather with 1..64 lanes per row.  Membership tests of k2k/k2c
+// (:328-447) use binary search: edge lists are sorted and unique
+// (loader/base_loader.hpp:481-498), so the result equals the linear scan's.
+// All kernels handle many queries at once through per-query descriptors.
+#include "kernels.hpp"
+
+namespace wk {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kPersistentGrid = 2048;
+
+__device__ __forceinline__ const Slot* bucket_addr(const StepArgs& a, const DevSeg& s, uint64_t b) {
+  const uint64_t mask = (uint64_t(1) << a.kshift) - 1;
+  const uint64_t blk = a.ktab[s.kb_base + (b >> a.kshift)];
+  return a.karena + ((blk << a.kshift) | (b & mask)) * ASSOCIATIVITY;
+}
+
+__device__ __forceinline__ uint32_t edge_at(const StepArgs& a, const DevSeg& s, uint64_t e) {
+  const uint64_t mask = (uint64_t(1) << a.vshift) - 1;
+  const uint64_t blk = a.vtab[s.vb_base + (e >> a.vshift)];
+  return a.varena[(blk << a.vshift) | (e & mask)];
+}
+
+// Largest q in [0, nq) with rs[q] <= g (queries with no rows are skipped).
+__device__ __forceinline__ uint32_t find_query(const uint64_t* rs, uint32_t nq, uint64_t g) {
+  uint32_t lo = 0, hi = nq;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rs[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const T y = __shfl_up(x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan over the block; every thread must call it.
+template <int NT>
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* total) {
+  constexpr int NW = NT / kWave;
+  __shared__ uint64_t wsum[NW];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint64_t x = wave_incl_scan<unsigned long long>(v);
+  if (lane == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    uint64_t s = lane < NW ? wsum[lane] : 0;
+    s = wave_incl_scan<unsigned long long>(s);
+    if (lane < NW) wsum[lane] = s;
+  }
+  __syncthreads();
+  const uint64_t wbase = w ? wsum[w - 1] : 0;
+  if (total) *total = wsum[NW - 1];
+  __syncthreads();
+  return wbase + x - v;
+}
+
+template <int NT>
+__device__ __forceinline__ uint64_t block_sum(uint64_t v) {
+  uint64_t t;
+  block_excl_scan<NT>(v, &t);
+  return t;
+}
+
+// Cluster-chained probe of one 128-byte bucket per hop by W cooperating
+// lanes (KVStore::get_slot_local, core/store/kvstore.hpp:292-314): slots 0..6
+// are compared with the key, slot 7 links to the next bucket or is empty.
+template <int W>
+__device__ __forceinline__ bool probe(const StepArgs& a, const DevSeg& s, uint64_t key, uint64_t& ptr) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gl = lane & (W - 1);
+  const int gbase = lane & ~(W - 1);
+  uint64_t b = hash_u64(key) % s.num_buckets;
+  for (uint64_t hop = 0; hop < s.total_buckets; ++hop) {
+    const uint4* bk = reinterpret_cast<const uint4*>(bucket_addr(a, s, b));
+    bool hit = false;
+    uint64_t hptr = 0, link = 0;
+#pragma unroll
+    for (int k = 0; k < ASSOCIATIVITY / W; ++k) {
+      const int si = gl + k * W;
+      const uint4 v = bk[si];
+      const uint64_t sk = uint64_t(v.x) | (uint64_t(v.y) << 32);
+      if (si < ASSOCIATIVITY - 1) {
+        if (sk == key) {
+          hit = true;
+          hptr = uint64_t(v.z) | (uint64_t(v.w) << 32);
+        }
+      } else {
+        link = sk;
+      }
+    }
+    if constexpr (W == 1) {
+      if (hit) {
+        ptr = hptr;
+        return true;
+      }
+    } else {
+      const uint64_t m = (__ballot(hit) >> gbase) & ((uint64_t(1) << W) - 1);
+      if (m) {
+        const int src = gbase + __ffsll((long long)m) - 1;
+        ptr = __shfl(hptr, src, kWave);
+        return true;
+      }
+      link = __shfl(link, gbase + ((ASSOCIATIVITY - 1) & (W - 1)), kWave);
+    }
+    if (link == 0) return false;
+    b = key_vid(link);
+    if (b >= s.total_buckets) return false;  // malformed link: treat as absent
+  }
+  return false;
+}
+
+// Membership of x in the sorted edge list [off, off+n) of segment s.
+__device__ __forceinline__ bool contains(const StepArgs& a, const DevSeg& s, uint64_t off, uint64_t n,
+                                         uint32_t x) {
+  uint64_t lo = 0, hi = n;
+  while (hi - lo > 8) {
+    const uint64_t mid = (lo + hi) >> 1;
+    const uint32_t v = edge_at(a, s, off + mid);
+    if (v < x) lo = mid + 1;
+    else if (v > x) hi = mid;
+    else return true;
+  }
+  for (uint64_t i = lo; i < hi; ++i)
+    if (edge_at(a, s, off + i) == x) return true;
+  return false;
+}
+
+__device__ __forceinline__ uint64_t step_rows(const StepArgs& a) {
+  const uint64_t r = a.row_start[a.nq];
+  return r < a.rcap ? r : a.rcap;
+}
+
+// ---- probe: per input row, edge count (expand) or keep flag (filters) -----
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  const uint64_t ntiles = (R + kTile - 1) / kTile;
+  constexpr int G = kBlock / W;
+  const int grp = threadIdx.x / W, gl = threadIdx.x % W;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    uint64_t tsum = 0;
+    for (int pass = 0; pass < kTile / G; ++pass) {
+      const uint64_t g = t * kTile + uint64_t(pass) * G + grp;
+      if (g < R) {
+        uint32_t c = 0, eo = 0;
+        const uint32_t q = find_query(a.row_start, a.nq, g);
+        const StepDesc& d = a.desc[q];
+        if (d.op != OP_SKIP && d.seg >= 0) {
+          const DevSeg s = a.segs[d.seg];
+          const uint32_t* row = a.tin + d.in_off + (g - a.row_start[q]) * d.in_cols;
+          uint64_t key = 0;
+          bool valid = s.resident != 0;
+          if (d.key_mode == KEY_CONST) {
+            key = make_key(d.kstart, d.pid, d.dir);
+          } else if (d.key_mode == KEY_TYPE_INDEX) {
+            // [0|t|IN]: only type/predicate ids fit the 17-bit pid field
+            const uint32_t v = row[d.sc];
+            valid = valid && v < (1u << NBITS_IDX);
+            key = make_key(0, v, DIR_IN);
+          } else {
+            key = make_key(row[d.sc], d.pid, d.dir);
+          }
+          uint64_t ptr = 0;
+          if (valid && probe<W>(a, s, key, ptr)) {
+            const uint64_t sz = ptr_size(ptr);
+            const uint64_t off = ptr_off(ptr) - s.edge_start;
+            if (d.op == OP_EXPAND) {
+              c = uint32_t(sz);
+              eo = uint32_t(off);
+            } else {
+              const uint32_t target = (d.op == OP_FILTER_KNOWN) ? row[d.ec] : d.kend;
+              c = contains(a, s, off, sz, target) ? 1u : 0u;
+            }
+          }
+        }
+        if (gl == 0) {
+          a.cnt[g] = c;
+          a.eoff[g] = eo;
+          tsum += c;
+        }
+      }
+    }
+    tsum = block_sum<kBlock>(tsum);
+    if (threadIdx.x == 0 && t < a.tile_cap) a.tile_sum[t] = tsum;
+  }
+}
+
+// ---- scan -----------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  uint64_t nt = (R + kTile - 1) / kTile;
+  if (nt > a.tile_cap) nt = a.tile_cap;
+  const uint64_t per = (nt + 1023) / 1024;
+  uint64_t lo = uint64_t(threadIdx.x) * per;
+  if (lo > nt) lo = nt;
+  const uint64_t hi = (lo + per < nt) ? lo + per : nt;
+  uint64_t s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += a.tile_sum[i];
+  uint64_t total = 0;
+  uint64_t ex = block_excl_scan<1024>(s, &total);
+  for (uint64_t i = lo; i < hi; ++i) {
+    const uint64_t v = a.tile_sum[i];
+    a.tile_sum[i] = ex;
+    ex += v;
+  }
+  if (threadIdx.x == 0) a.st->total_out_rows = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  uint64_t nt = (R + kTile - 1) / kTile;
+  if (nt > a.tile_cap) nt = a.tile_cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.pfx[R] = a.st->total_out_rows;
+  for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const uint64_t g0 = t * kTile + uint64_t(threadIdx.x) * kRowsPerThread;
+    uint32_t c[kRowsPerThread];
+    uint64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kRowsPerThread; ++j) {
+      c[j] = (g0 + j < R) ? a.cnt[g0 + j] : 0u;
+      s += c[j];
+    }
+    uint64_t ex = block_excl_scan<kBlock>(s, nullptr) + a.tile_sum[t];
+#pragma unroll
+    for (int j = 0; j < kRowsPerThread; ++j) {
+      const uint64_t g = g0 + j;
+      if (g < R) {
+        a.pfx[g] = ex;
+        if (c[j]) {
+          // this row holds the first output row of every tile starting in [ex, ex+c)
+          uint64_t tt = (ex + kOutTile - 1) / kOutTile;
+          const uint64_t last = (ex + c[j] - 1) / kOutTile;
+          for (; tt <= last && tt < a.otile_cap; ++tt) a.tile_row[tt] = g;
+        }
+        ex += c[j];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  uint64_t carry_t = a.st->tout_base, carry_a = a.st->arena_base, carry_n = 0;
+  for (uint32_t base = 0; base < a.nq; base += 1024) {
+    const uint32_t q = base + threadIdx.x;
+    uint64_t et = 0, ea = 0, nr = 0, rows = 0;
+    if (q < a.nq) {
+      const StepDesc& d = a.desc[q];
+      uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
+      r0 = r0 < R ? r0 : R;
+      r1 = r1 < R ? r1 : R;
+      rows = a.pfx[r1] - a.pfx[r0];
+      const uint64_t el = rows * d.out_cols;
+      if (d.final_step) {
+        ea = el;
+      } else {
+        et = el;
+        nr = rows;
+      }
+    }
+    uint64_t tt = 0, ta = 0, tn = 0;
+    const uint64_t xt = block_excl_scan<1024>(et, &tt);
+    const uint64_t xa = block_excl_scan<1024>(ea, &ta);
+    const uint64_t xn = block_excl_scan<1024>(nr, &tn);
+    if (q < a.nq) {
+      StepDesc& d = a.desc[q];
+      d.out_rows = rows;
+      d.out_off = d.final_step ? carry_a + xa : carry_t + xt;
+      if (a.next) {
+        a.next[q].in_rows = nr;
+        a.next[q].in_off = carry_t + xt;
+        a.next_row_start[q] = carry_n + xn;
+      }
+    }
+    carry_t += tt;
+    carry_a += ta;
+    carry_n += tn;
+  }
+  if (threadIdx.x == 0) {
+    uint32_t status = 0;
+    if (carry_t > a.tcap) status = 1;
+    if (carry_a > a.arena_cap) status = 2;
+    a.st->out_elems = carry_t - a.st->tout_base;
+    a.st->status = status;
+    if (a.next) {
+      a.next_row_start[a.nq] = carry_n;
+      if (a.next_st) {
+        a.next_st->arena_base = carry_a;
+        a.next_st->tout_base = 0;
+        a.next_st->status = status;
+      }
+    }
+  }
+}
+
+// ---- gather ---------------------------------------------------------------
+
+__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
+  const StepDesc& d = a.desc[q];
+  const uint64_t r0 = a.row_start[q];
+  const uint64_t orow = j - a.pfx[r0];
+  uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
+  const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
+  const uint32_t ic = d.in_cols;
+  for (uint32_t c = 0; c < ic; ++c) dst[c] = src[c];
+  if (d.op == OP_EXPAND) {
+    const DevSeg s = a.segs[d.seg];
+    dst[ic] = edge_at(a, s, uint64_t(a.eoff[g]) + (j - a.pfx[g]));
+  }
+}
+
+// Output-balanced: one output row per thread; the tile -> first-row map from
+// scan_rows bounds each thread's binary search for its source row.
+__global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  const uint64_t total = a.st->total_out_rows;
+  uint64_t nt = (total + kOutTile - 1) / kOutTile;
+  if (nt > a.otile_cap) nt = a.otile_cap;
+  for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const uint64_t j = t * kOutTile + threadIdx.x;
+    if (j >= total) continue;
+    uint64_t lo = a.tile_row[t];
+    uint64_t hi = (t + 1 < nt) ? a.tile_row[t + 1] + 1 : R;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (a.pfx[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    emit_row(a, find_query(a.row_start, a.nq, lo), lo, j);
+  }
+}
+
+// Row-centric: LPR lanes share one input row (64/LPR rows per wavefront).
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  constexpr int G = kBlock / LPR;
+  const int grp = threadIdx.x / LPR, gl = threadIdx.x % LPR;
+  for (uint64_t g = uint64_t(blockIdx.x) * G + grp; g < R; g += uint64_t(gridDim.x) * G) {
+    const uint32_t c = a.cnt[g];
+    if (!c) continue;
+    const uint32_t q = find_query(a.row_start, a.nq, g);
+    const uint64_t p = a.pfx[g];
+    for (uint32_t k = gl; k < c; k += LPR) emit_row(a, q, g, p + k);
+  }
+}
+
+// Per-table checksums (sum of row hashes) of non-empty tables whose rows are
+// numbered consecutively; a block's 256 rows span at most 256 tables.
+__global__ __launch_bounds__(kBlock) void k_checksum(const uint32_t* base, const CkDesc* d, uint32_t n,
+                                                     uint64_t total, unsigned long long* sums) {
+  __shared__ unsigned long long acc[kBlock];
+  for (uint64_t t0 = uint64_t(blockIdx.x) * kBlock; t0 < total; t0 += uint64_t(gridDim.x) * kBlock) {
+    acc[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (d[mid].row0 <= t0) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t qa = lo;
+    const uint64_t r = t0 + threadIdx.x;
+    if (r < total) {
+      uint32_t a = qa, b = n;
+      while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (d[mid].row0 <= r) a = mid;
+        else b = mid;
+      }
+      const CkDesc& c = d[a];
+      const uint64_t h = row_hash(base + c.off + (r - c.row0) * c.cols, c.cols);
+      atomicAdd(&acc[a - qa], (unsigned long long)h);
+    }
+    __syncthreads();
+    if (qa + threadIdx.x < n && acc[threadIdx.x]) atomicAdd(&sums[qa + threadIdx.x], acc[threadIdx.x]);
+    __syncthreads();
+  }
+}
+
+// Checksums of the final tables of one step: table f = desc[idx[f]] in the
+// arena, handled by `chunks` blocks.
+__global__ __launch_bounds__(kBlock) void k_checksum_final(const uint32_t* arena, const StepDesc* desc,
+                                                           const uint32_t* idx, uint32_t n, uint32_t chunks,
+                                                           const StepState* st, unsigned long long* sums) {
+  if (st->status) return;
+  const uint32_t f = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+  if (f >= n) return;
+  const StepDesc& d = desc[idx[f]];
+  uint64_t s = 0;
+  for (uint64_t r = uint64_t(ch) * kBlock + threadIdx.x; r < d.out_rows; r += uint64_t(chunks) * kBlock)
+    s += row_hash(arena + d.out_off + r * d.out_cols, d.out_cols);
+  s = block_sum<kBlock>(s);
+  if (threadIdx.x == 0 && s) atomicAdd(&sums[f], (unsigned long long)s);
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+inline unsigned grid_for(uint64_t units, unsigned cap) {
+  if (units == 0) return 1;
+  return unsigned(units < cap ? units : cap);
+}
+
+}  // namespace
+
+void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
+  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
+  switch (w) {
+    case 1: hipLaunchKernelGGL(k_probe<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_probe<2>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_probe<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_probe<8>, dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, a);
+  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
+  hipLaunchKernelGGL(k_scan_rows, dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
+}
+
+void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
+  if (rows_per_wave <= 0) {
+    const unsigned grid = out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : 4 * kPersistentGrid;
+    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(kBlock), 0, s, a);
+    return;
+  }
+  // rows per wavefront -> lanes per row
+  const unsigned grid = kPersistentGrid * 2;
+  switch (rows_per_wave) {
+    case 64: hipLaunchKernelGGL(k_gather_rows<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(k_gather_rows<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_gather_rows<64>, dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t total_rows, uint64_t* sums,
+                     hipStream_t s) {
+  if (n == 0 || total_rows == 0) return;
+  hipLaunchKernelGGL(k_checksum, dim3(grid_for((total_rows + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
+                     base, d, n, total_rows, reinterpret_cast<unsigned long long*>(sums));
+}
+
+void launch_checksum_final(const uint32_t* arena, const StepDesc* desc, const uint32_t* idx, uint32_t n,
+                           const StepState* st, uint64_t* sums, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t chunks = 16;
+  hipLaunchKernelGGL(k_checksum_final, dim3(n * chunks), dim3(kBlock), 0, s, arena, desc, idx, n, chunks, st,
+                     reinterpret_cast<unsigned long long*>(sums));
+}
+
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((n + 255) / 256, 4096)), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace wk

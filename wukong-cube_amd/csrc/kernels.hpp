@@ -1,0 +1,123 @@
+// Structures shared by the device kernels and the host orchestration, and the
+// host-side launch wrappers of the expansion kernels.
+//
+// One pattern step of one or many queries runs as:
+//   probe        -> per input row: edge count (k2u) or keep flag (k2k/k2c)
+//   scan_tiles   -> exclusive scan of per-tile sums
+//   scan_rows    -> per-row 64-bit output offsets + output-tile -> row map
+//   scan_queries -> per-query output rows/offsets, next step's row space
+//   gather       -> write the next history table(s)
+// All sizes live in device memory, so a multi-step batch can be captured in a
+// hipGraph with no host round trip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "wk_common.hpp"
+
+namespace wk {
+
+enum : uint32_t { OP_EXPAND = 0, OP_FILTER_KNOWN = 1, OP_FILTER_CONST = 2, OP_SKIP = 3 };
+enum : uint32_t { KEY_FROM_ROW = 0, KEY_CONST = 1, KEY_TYPE_INDEX = 2 };
+
+// Segment descriptor as seen by the kernels (device array, one per segment).
+struct DevSeg {
+  uint64_t num_buckets;  // main buckets: hash modulus
+  uint64_t total_buckets;// main + extended: bound on chain links
+  uint64_t edge_start;   // global edge offset of the segment (ptr.off - edge_start = local)
+  uint64_t num_edges;
+  uint32_t kb_base;      // first entry in the key-block table
+  uint32_t vb_base;      // first entry in the value-block table
+  uint32_t resident;
+  uint32_t pad;
+};
+
+// One query's work in one pattern step.
+struct StepDesc {
+  uint32_t op;
+  int32_t seg;           // segment index, -1: key cannot exist (no rows)
+  uint32_t pid, dir;
+  uint32_t key_mode;
+  uint32_t sc, ec;       // start / end column
+  uint32_t kstart;       // constant start vid (KEY_CONST)
+  uint32_t kend;         // constant end (k2c)
+  uint32_t in_cols, out_cols;
+  uint32_t final_step;   // 1: write into the result arena
+  uint64_t in_off;       // element offset of the input table
+  uint64_t in_rows;
+  uint64_t out_rows;     // set by scan_queries
+  uint64_t out_off;      // set by scan_queries (element offset in its destination)
+};
+
+// Per-step shared state (device).
+struct StepState {
+  uint64_t total_out_rows;  // output rows over all queries (set by scan_tiles)
+  uint64_t tout_base;       // first element of the ping-pong out table to use
+  uint64_t out_elems;       // elements written to the out table (set by scan_queries)
+  uint64_t arena_base;      // first free element of the result arena
+  uint32_t status;          // 0 ok, 1 table overflow, 2 arena overflow
+  uint32_t pad;
+};
+
+// Pointers and capacities used by every step kernel.
+struct StepArgs {
+  const Slot* karena;
+  const uint32_t* varena;
+  const uint32_t* ktab;
+  const uint32_t* vtab;
+  const DevSeg* segs;
+  uint32_t kshift, vshift;
+  const uint32_t* tin;     // input table buffer
+  uint32_t* tout;          // ping-pong output buffer
+  uint32_t* arena;         // result arena (final steps)
+  uint64_t tcap;           // elements of tout
+  uint64_t arena_cap;      // elements of arena
+  StepDesc* desc;          // [nq] this step
+  StepDesc* next;          // [nq] next step (rows/offsets filled by scan_queries) or null
+  uint64_t* row_start;     // [nq+1] flattened row starts of this step
+  uint64_t* next_row_start;// [nq+1] of the next step or null
+  uint32_t nq;
+  uint32_t* cnt;           // [rcap]
+  uint32_t* eoff;          // [rcap]
+  uint64_t* pfx;           // [rcap+1]
+  uint64_t rcap;
+  uint64_t* tile_sum;      // [tile_cap]
+  uint64_t tile_cap;
+  uint64_t* tile_row;      // [otile_cap+1] first source row of each output tile
+  uint64_t otile_cap;
+  StepState* st;
+  StepState* next_st;      // or null
+};
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kRowsPerThread = 4;
+constexpr int kTile = kBlock * kRowsPerThread;  // rows per probe/scan tile
+constexpr int kOutTile = kBlock;                // output rows per gather tile
+
+// Launch wrappers (all on `stream`; grids sized from host hints, kernels
+// re-read the device sizes and stride).  `rows_hint`/`out_hint` = 0 means
+// unknown on the host: a fixed persistent grid is used.
+void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s);
+void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s);
+void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+
+// One non-empty table for the checksum kernel: `rows` rows of `cols`
+// elements at element offset `off`; rows of all tables numbered from `row0`.
+struct CkDesc {
+  uint64_t off;
+  uint64_t rows;
+  uint64_t row0;
+  uint32_t cols;
+  uint32_t pad;
+};
+// sums[i] += checksum of table i (sums must be zeroed by the caller).
+void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t total_rows, uint64_t* sums,
+                     hipStream_t s);
+// sums[f] += checksum of the final table desc[idx[f]] in the arena.
+void launch_checksum_final(const uint32_t* arena, const StepDesc* desc, const uint32_t* idx, uint32_t n,
+                           const StepState* st, uint64_t* sums, hipStream_t s);
+
+}  // namespace wk

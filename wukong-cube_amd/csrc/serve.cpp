@@ -1,0 +1,351 @@
+// Serving: open-loop replay of an arrival trace with continuous batching.
+//
+// The model is the reference's emulator and engine loop (run_query_emu,
+// src/client/console_proxy.hpp:287-439; Engine::run, core/engine/engine.hpp:132-233)
+// moved onto the device: every scheduler iteration advances each in-flight
+// query by one pattern step in one set of launches; queries that arrived are
+// admitted whenever an in-flight slot and projected table footprint are free,
+// and finished queries leave at the end of the iteration that completed them.
+#include <algorithm>
+#include <chrono>
+#include <thread>
+
+#include "device.hpp"
+
+namespace wk {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct QState {
+  uint32_t k = 0;      // next step
+  uint64_t rows = 0;   // rows of the current table
+  uint64_t off = 0;    // element offset of the current table in the input buffer
+  double admit = 0;
+};
+
+// Projected history-table bytes of a query: exact first-pattern cardinality
+// from the host store, then the average degree of each expanded segment.
+uint64_t project_footprint(const Store& st, const Plan& p) {
+  double rows = 1, best = 0;
+  for (size_t k = 0; k < p.steps.size(); k++) {
+    const StepDesc& sd = p.steps[k];
+    const double in = rows;
+    if (sd.op == OP_EXPAND) {
+      if (sd.key_mode == KEY_CONST) {
+        uint64_t sz = 0;
+        st.get_triples(sd.kstart, sd.pid, int(sd.dir), sz);
+        rows = double(sz);
+      } else if (sd.seg >= 0) {
+        const SegMeta& m = st.segs[size_t(sd.seg)];
+        rows = in * (m.num_keys ? double(m.num_edges) / double(m.num_keys) : 0.0);
+      } else {
+        rows = 0;
+      }
+    }
+    const double bytes = (in * sd.in_cols + rows * sd.out_cols) * 4.0 + in * 16.0;
+    best = std::max(best, bytes);
+  }
+  return uint64_t(best) + 64;
+}
+
+struct Server {
+  wk_dev* dev;
+  const std::vector<Plan>& plans;
+  std::vector<QState>& qs;
+  uint32_t* d_idx = nullptr;
+  uint64_t* d_sums = nullptr;
+  uint32_t* h_idx = nullptr;
+  uint64_t* h_sums = nullptr;
+  uint32_t cap = 0;
+
+  Server(wk_dev* d, const std::vector<Plan>& p, std::vector<QState>& s) : dev(d), plans(p), qs(s) {}
+  ~Server() {
+    if (d_idx) (void)hipFree(d_idx);
+    if (d_sums) (void)hipFree(d_sums);
+    if (h_idx) (void)hipHostFree(h_idx);
+    if (h_sums) (void)hipHostFree(h_sums);
+  }
+  int reserve(uint32_t n) {
+    if (n <= cap) return WK_OK;
+    if (d_idx) (void)hipFree(d_idx);
+    if (d_sums) (void)hipFree(d_sums);
+    if (h_idx) (void)hipHostFree(h_idx);
+    if (h_sums) (void)hipHostFree(h_sums);
+    cap = std::max<uint32_t>(n, 1024);
+    WK_HIP(hipMalloc(&d_idx, cap * sizeof(uint32_t)));
+    WK_HIP(hipMalloc(&d_sums, cap * sizeof(uint64_t)));
+    WK_HIP(hipHostMalloc(&h_idx, cap * sizeof(uint32_t), hipHostMallocDefault));
+    WK_HIP(hipHostMalloc(&h_sums, cap * sizeof(uint64_t), hipHostMallocDefault));
+    return WK_OK;
+  }
+
+  // One step for queries F[a, b): outputs from tout_base / arena_base.  On
+  // overflow the range is halved; a single query that cannot fit fails.
+  // `res` receives the step descriptor of each query after the step.
+  int step_range(const std::vector<uint32_t>& F, size_t a, size_t b, int cur, uint64_t tout_base,
+                 uint64_t arena_base, uint64_t* used_t, uint64_t* used_a, std::vector<StepDesc>& res,
+                 std::vector<int>& failed, bool checksum_now) {
+    const uint32_t n = uint32_t(b - a);
+    *used_t = tout_base;
+    *used_a = arena_base;
+    if (n == 0) return WK_OK;
+    int rc = reserve_steps(dev, 1, n);
+    if (rc) return rc;
+    rc = reserve(n);
+    if (rc) return rc;
+    std::vector<int> segs;
+    uint64_t R = 0;
+    uint32_t nfinal = 0;
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t q = F[a + i];
+      const Plan& p = plans[q];
+      StepDesc d = p.steps[qs[q].k];
+      d.final_step = (qs[q].k + 1 == p.steps.size()) ? 1 : 0;
+      d.in_rows = qs[q].k == 0 ? 1 : qs[q].rows;
+      d.in_off = qs[q].k == 0 ? 0 : qs[q].off;
+      dev->h_desc[i] = d;
+      dev->h_rs[i] = R;
+      R += d.in_rows;
+      if (d.final_step) h_idx[nfinal++] = i;
+      if (d.seg >= 0 && std::find(segs.begin(), segs.end(), d.seg) == segs.end()) segs.push_back(d.seg);
+    }
+    dev->h_rs[n] = R;
+    if (R > dev->rcap) {
+      if (n == 1) {
+        failed.push_back(int(a));
+        return WK_OK;
+      }
+      goto split;
+    }
+    {
+      dev->h_state[0] = StepState{};
+      dev->h_state[0].tout_base = tout_base;
+      dev->h_state[0].arena_base = arena_base;
+      rc = ensure_segments(dev, segs);
+      if (rc) return rc;
+      const uint32_t* tin = dev->tbuf[cur];
+      uint32_t* tout = dev->tbuf[cur ^ 1];
+      if (hipMemcpyAsync(dev->d_desc, dev->h_desc, n * sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream) ||
+          hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream) ||
+          hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream) ||
+          hipMemcpyAsync(d_idx, h_idx, std::max<uint32_t>(nfinal, 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         dev->stream)) {
+        unpin_segments(dev, segs);
+        return WK_E_HIP;
+      }
+      const StepArgs args = make_args(dev, 0, n, tin, tout, dev->tcap, false);
+      launch_probe(args, dev->opts.probe_width, R, dev->stream);
+      launch_scan(args, R, dev->stream);
+      launch_gather(args, dev->opts.rows_per_wave, 0, dev->stream);
+      if (checksum_now && nfinal) {
+        launch_fill_u64(d_sums, nfinal, 0, dev->stream);
+        launch_checksum_final(dev->arena, dev->d_desc, d_idx, nfinal, dev->d_state, d_sums, dev->stream);
+        if (hipMemcpyAsync(h_sums, d_sums, nfinal * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream)) {
+          unpin_segments(dev, segs);
+          return WK_E_HIP;
+        }
+      }
+      if (hipMemcpyAsync(dev->h_desc, dev->d_desc, n * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream) ||
+          hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream) ||
+          hipStreamSynchronize(dev->stream)) {
+        unpin_segments(dev, segs);
+        return WK_E_HIP;
+      }
+      unpin_segments(dev, segs);
+      if (dev->h_state[0].status == 0) {
+        for (uint32_t i = 0; i < n; i++) res[a + i] = dev->h_desc[i];
+        *used_t = tout_base + dev->h_state[0].out_elems;
+        uint64_t ua = arena_base;
+        for (uint32_t i = 0; i < n; i++)
+          if (dev->h_desc[i].final_step)
+            ua = std::max<uint64_t>(ua, dev->h_desc[i].out_off + dev->h_desc[i].out_rows * dev->h_desc[i].out_cols);
+        *used_a = ua;
+        if (checksum_now)
+          for (uint32_t f = 0; f < nfinal; f++) res[a + h_idx[f]].in_off = h_sums[f];  // carry checksum
+        return WK_OK;
+      }
+      if (n == 1) {
+        failed.push_back(int(a));
+        return WK_OK;
+      }
+    }
+  split : {
+    const size_t m = a + n / 2;
+    uint64_t t1 = 0, a1 = 0;
+    rc = step_range(F, a, m, cur, tout_base, arena_base, &t1, &a1, res, failed, checksum_now);
+    if (rc) return rc;
+    return step_range(F, m, b, cur, t1, a1, used_t, used_a, res, failed, checksum_now);
+  }
+  }
+};
+
+}  // namespace
+}  // namespace wk
+
+using namespace wk;
+
+extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const double* arrival_s, uint32_t nq,
+                               const wk_serve_opts* opts, wk_serve_result* out) {
+  if (!dev || (!queries && nq) || (!arrival_s && nq) || !opts || !out) return WK_E_INVALID_ARG;
+  for (uint32_t i = 1; i < nq; i++)
+    if (arrival_s[i] < arrival_s[i - 1]) return WK_E_INVALID_ARG;
+  std::vector<Plan> plans(nq);
+  for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, queries[i], plans[i]);
+  std::vector<QState> qs(nq);
+  std::vector<uint64_t> fp(nq);
+  for (uint32_t i = 0; i < nq; i++) fp[i] = plans[i].status ? 0 : project_footprint(*dev->st, plans[i]);
+  const uint32_t max_inflight = opts->per_query ? 1 : std::max<uint32_t>(1, opts->max_inflight);
+  const uint64_t budget = opts->footprint_budget ? opts->footprint_budget : dev->tcap * 4;
+  if (opts->keep_results) {
+    dev->served_rows.assign(nq, {});
+    dev->served_cols.assign(nq, 0);
+  } else {
+    dev->served_rows.clear();
+    dev->served_cols.clear();
+  }
+  Server srv(dev, plans, qs);
+  const auto t0 = Clock::now();
+  auto now_s = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+  auto finish = [&](uint32_t q, double t, uint64_t rows, int status, uint64_t ck) {
+    out->latency_us[q] = (t - arrival_s[q]) * 1e6;
+    out->start_us[q] = (qs[q].admit - arrival_s[q]) * 1e6;
+    out->rows[q] = rows;
+    out->status[q] = status;
+    out->checksum[q] = ck;
+  };
+  auto keep = [&](uint32_t q, const uint32_t* dsrc, uint64_t rows, uint32_t cols) -> int {
+    if (!opts->keep_results) return WK_OK;
+    dev->served_rows[q].resize(rows * cols);
+    dev->served_cols[q] = cols;
+    if (rows * cols)
+      WK_HIP(hipMemcpy(dev->served_rows[q].data(), dsrc, rows * cols * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return WK_OK;
+  };
+  uint32_t next = 0, ndone = 0;
+  uint64_t batches = 0;
+  int cur = 0;
+  uint64_t fp_used = 0;
+  std::vector<uint32_t> inflight;
+  std::vector<StepDesc> res;
+  std::vector<int> failed;
+  CkDesc* d_ck = nullptr;
+  uint64_t* d_sum = nullptr;
+  if (opts->per_query) {
+    WK_HIP(hipMalloc(&d_ck, sizeof(CkDesc)));
+    WK_HIP(hipMalloc(&d_sum, sizeof(uint64_t)));
+  }
+  int rc = WK_OK;
+  while (ndone < nq && rc == WK_OK) {
+    double now = now_s();
+    while (next < nq && arrival_s[next] <= now && inflight.size() < max_inflight) {
+      const uint32_t q = next;
+      if (plans[q].status || plans[q].steps.empty()) {
+        qs[q].admit = now;
+        finish(q, now, 0, plans[q].status, 0);
+        if (opts->keep_results) dev->served_cols[q] = plans[q].cols;
+        ndone++;
+        next++;
+        continue;
+      }
+      if (!inflight.empty() && fp_used + fp[q] > budget) break;
+      qs[q] = QState{};
+      qs[q].admit = now;
+      fp_used += fp[q];
+      inflight.push_back(q);
+      next++;
+    }
+    if (inflight.empty()) {
+      if (next < nq) {
+        const double wait = arrival_s[next] - now_s();
+        if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+      }
+      continue;
+    }
+    batches++;
+    if (opts->per_query) {
+      // one query, pattern by pattern with a host sync per pattern
+      const uint32_t q = inflight[0];
+      uint64_t rows = 0;
+      int buf = -1;
+      int st = run_query_device(dev, plans[q], &rows, &buf);
+      uint64_t ck = 0;
+      if (st == WK_OK && rows) {
+        CkDesc c{0, rows, 0, plans[q].cols, 0};
+        if (hipMemcpyAsync(d_ck, &c, sizeof(c), hipMemcpyHostToDevice, dev->stream)) {
+          rc = WK_E_HIP;
+          break;
+        }
+        launch_fill_u64(d_sum, 1, 0, dev->stream);
+        launch_checksum(dev->tbuf[buf], d_ck, 1, rows, d_sum, dev->stream);
+        if (hipMemcpyAsync(&ck, d_sum, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream) ||
+            hipStreamSynchronize(dev->stream)) {
+          rc = WK_E_HIP;
+          break;
+        }
+        rc = keep(q, dev->tbuf[buf], rows, plans[q].cols);
+      } else if (opts->keep_results) {
+        dev->served_cols[q] = plans[q].cols;
+      }
+      if (st != WK_OK) rows = 0;
+      finish(q, now_s(), rows, st, ck);
+      fp_used -= fp[q];
+      inflight.clear();
+      ndone++;
+      continue;
+    }
+    res.assign(inflight.size(), StepDesc{});
+    failed.clear();
+    uint64_t used_t = 0, used_a = 0;
+    rc = srv.step_range(inflight, 0, inflight.size(), cur, 0, 0, &used_t, &used_a, res, failed, true);
+    if (rc) break;
+    const double t = now_s();
+    std::vector<uint32_t> still;
+    for (size_t i = 0; i < inflight.size(); i++) {
+      const uint32_t q = inflight[i];
+      if (std::find(failed.begin(), failed.end(), int(i)) != failed.end()) {
+        finish(q, t, 0, WK_E_NOMEM_BUDGET, 0);
+        if (opts->keep_results) dev->served_cols[q] = plans[q].cols;
+        fp_used -= fp[q];
+        ndone++;
+        continue;
+      }
+      const StepDesc& d = res[i];
+      if (d.final_step || d.out_rows == 0) {
+        // an empty table stays empty through the remaining patterns
+        uint64_t ck = d.final_step ? d.in_off : 0;
+        if (d.final_step) rc = keep(q, dev->arena + d.out_off, d.out_rows, d.out_cols);
+        else if (opts->keep_results) dev->served_cols[q] = plans[q].cols;
+        finish(q, t, d.final_step ? d.out_rows : 0, WK_OK, ck);
+        fp_used -= fp[q];
+        ndone++;
+      } else {
+        qs[q].rows = d.out_rows;
+        qs[q].off = d.out_off;
+        qs[q].k++;
+        still.push_back(q);
+      }
+    }
+    inflight.swap(still);
+    cur ^= 1;
+  }
+  if (d_ck) (void)hipFree(d_ck);
+  if (d_sum) (void)hipFree(d_sum);
+  out->wall_s = now_s();
+  out->batches = batches;
+  return rc;
+}
+
+extern "C" int wk_serve_result_rows(wk_dev* dev, uint32_t i, uint32_t* dst, uint64_t cap_elems, uint64_t* rows,
+                                    uint32_t* cols) {
+  if (!dev || i >= dev->served_cols.size()) return WK_E_STATE;
+  const uint32_t c = dev->served_cols[i];
+  const uint64_t n = dev->served_rows[i].size();
+  if (rows) *rows = c ? n / c : 0;
+  if (cols) *cols = c;
+  if (dst) {
+    if (cap_elems < n) return WK_E_INVALID_ARG;
+    std::copy(dev->served_rows[i].begin(), dev->served_rows[i].end(), dst);
+  }
+  return WK_OK;
+}

@@ -1,0 +1,503 @@
+"""Python binding of the MI355X Wukong GPU query path (ctypes over include/wk_gpu.h).
+
+The host engine, store builder and kernels are native (libwukong_gpu.so);
+this module only marshals arrays.  Loading fails loudly when the library has
+not been built: there is no Python fallback.
+
+Mirrors the reference's GPU engine flow: a `Store` is the host gstore
+(SegmentRDFGraph), a `Device` owns the HBM gstore cache and history tables
+(GPUMem + GPUCache + GPUEngine), `Device.query` runs a planned pattern list
+pattern by pattern (GPUAgent::execute_sparql_query).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwukong_gpu.so")
+
+IN, OUT = 0, 1
+PREDICATE_ID, TYPE_ID = 0, 1
+MAX_VARS = 64
+
+STATUS = {
+    0: "WK_OK", 1: "WK_E_UNKNOWN", 2: "WK_E_SYNTAX", 3: "WK_E_UNKNOWN_PATTERN",
+    4: "WK_E_ATTR_DISABLE", 5: "WK_E_NO_REQUIRED_VAR", 6: "WK_E_UNSUPPORT_UNION",
+    7: "WK_E_OBJ_ERROR", 8: "WK_E_VERTEX_INVALID", 9: "WK_E_UNKNOWN_SUB",
+    10: "WK_E_SETTING", 11: "WK_E_FIRST_PATTERN", 12: "WK_E_UNKNOWN_FILTER",
+    13: "WK_E_FILE_NOT_FOUND", 100: "WK_E_INVALID_ARG", 101: "WK_E_INVALID_TRIPLE",
+    102: "WK_E_NOMEM_HOST", 103: "WK_E_NOMEM_BUDGET", 104: "WK_E_HIP",
+    105: "WK_E_NO_DEVICE", 106: "WK_E_OVERFLOW", 107: "WK_E_SEG_TOO_LARGE",
+    108: "WK_E_STATE", 109: "WK_E_TOO_MANY_VARS",
+}
+globals().update({v: k for k, v in STATUS.items()})
+
+
+class WkError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = "%s (%d): %s" % (STATUS.get(code, "?"), code, _strerror(code))
+        if code == 104:
+            msg += " [hip error %d]" % lib().wk_last_hip_error()
+        super().__init__(msg + (" in " + what if what else ""))
+
+
+class Pattern(C.Structure):
+    _fields_ = [("subject", C.c_int32), ("predicate", C.c_int32), ("object", C.c_int32),
+                ("direction", C.c_int32)]
+
+
+class Query(C.Structure):
+    _fields_ = [("patterns", C.POINTER(Pattern)), ("npatterns", C.c_uint32), ("nvars", C.c_int32)]
+
+
+class StoreOpts(C.Structure):
+    _fields_ = [("pinned", C.c_int32), ("load_factor_pct", C.c_int32)]
+
+
+class StoreInfo(C.Structure):
+    _fields_ = [("num_triples_out", C.c_uint64), ("num_triples_in", C.c_uint64),
+                ("num_keys", C.c_uint64), ("num_buckets", C.c_uint64), ("num_values", C.c_uint64),
+                ("num_segments", C.c_uint32), ("pinned", C.c_int32)]
+
+
+class SegInfo(C.Structure):
+    _fields_ = [("index", C.c_uint32), ("pid", C.c_uint32), ("dir", C.c_uint32), ("pad", C.c_uint32),
+                ("num_keys", C.c_uint64), ("num_buckets", C.c_uint64), ("num_ext_buckets", C.c_uint64),
+                ("bucket_start", C.c_uint64), ("num_edges", C.c_uint64), ("edge_start", C.c_uint64)]
+
+
+class DevOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("gstore_budget", C.c_uint64), ("table_budget", C.c_uint64),
+                ("key_block_shift", C.c_uint32), ("value_block_shift", C.c_uint32),
+                ("enable_pipeline", C.c_int32), ("probe_width", C.c_int32),
+                ("rows_per_wave", C.c_int32), ("profile", C.c_int32)]
+
+
+class DevStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "h2d_bytes", "segments_loaded", "segments_evicted", "key_blocks_total", "key_blocks_free",
+        "value_blocks_total", "value_blocks_free", "table_capacity_elems", "queries_run")]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double),
+                ("alg_bytes", C.c_double)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("rows", C.POINTER(C.c_uint64)), ("cols", C.POINTER(C.c_uint32)),
+                ("status", C.POINTER(C.c_int32)), ("offset", C.POINTER(C.c_uint64)),
+                ("v2c", C.POINTER(C.c_int32))]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("max_inflight", C.c_uint32), ("footprint_budget", C.c_uint64),
+                ("per_query", C.c_int32), ("keep_results", C.c_int32)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("latency_us", C.POINTER(C.c_double)), ("start_us", C.POINTER(C.c_double)),
+                ("rows", C.POINTER(C.c_uint64)), ("status", C.POINTER(C.c_int32)),
+                ("checksum", C.POINTER(C.c_uint64)), ("wall_s", C.c_double),
+                ("batches", C.c_uint64)]
+
+
+_LIB = None
+P = C.c_void_p
+U32P = C.POINTER(C.c_uint32)
+U64P = C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes); every symbol declared in include/wk_gpu.h
+SIGNATURES = {
+    "wk_store_build": (C.c_int, [U32P, C.c_uint64, U32P, C.c_uint64, C.POINTER(StoreOpts), C.POINTER(P)]),
+    "wk_store_destroy": (None, [P]),
+    "wk_store_info_get": (C.c_int, [P, C.POINTER(StoreInfo)]),
+    "wk_store_segment": (C.c_int, [P, C.c_uint32, C.POINTER(SegInfo)]),
+    "wk_store_raw": (C.c_int, [P, C.POINTER(P), U64P, C.POINTER(U32P), U64P]),
+    "wk_store_get_triples": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(U32P), U64P]),
+    "wk_store_get_index": (C.c_int, [P, C.c_uint32, C.c_int32, C.POINTER(U32P), U64P]),
+    "wk_dev_create": (C.c_int, [P, C.POINTER(DevOpts), C.POINTER(P)]),
+    "wk_dev_destroy": (None, [P]),
+    "wk_dev_stats_get": (C.c_int, [P, C.POINTER(DevStats)]),
+    "wk_dev_kernel_stats": (C.c_int, [P, C.POINTER(KernelStat), C.c_uint32, U32P]),
+    "wk_dev_reset_kernel_stats": (C.c_int, [P]),
+    "wk_dev_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32]),
+    "wk_dev_sync": (C.c_int, [P]),
+    "wk_dev_load_all": (C.c_int, [P]),
+    "wk_seg_ensure": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
+    "wk_seg_resident": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32)]),
+    "wk_table_upload": (C.c_int, [P, U32P, C.c_uint64, C.c_uint32, C.POINTER(P)]),
+    "wk_table_shape": (C.c_int, [P, U64P, U32P]),
+    "wk_table_download": (C.c_int, [P, P, U32P, C.c_uint64]),
+    "wk_table_free": (C.c_int, [P, P]),
+    "wk_start_i2u": (C.c_int, [P, C.c_uint32, C.c_int32, C.POINTER(P)]),
+    "wk_start_c2u": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(P)]),
+    "wk_expand_k2u": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(P)]),
+    "wk_expand_k2k": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, C.POINTER(P)]),
+    "wk_expand_k2c": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, C.POINTER(P)]),
+    "wk_query_run": (C.c_int, [P, C.POINTER(Query), U64P, U32P, C.POINTER(C.c_int32)]),
+    "wk_result_download": (C.c_int, [P, U32P, C.c_uint64]),
+    "wk_batch_run": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(BatchResult)]),
+    "wk_batch_download": (C.c_int, [P, U32P, C.c_uint64, U64P]),
+    "wk_serve_replay": (C.c_int, [P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
+                                  C.POINTER(ServeOpts), C.POINTER(ServeResult)]),
+    "wk_serve_result_rows": (C.c_int, [P, C.c_uint32, U32P, C.c_uint64, U64P, U32P]),
+    "wk_table_checksum": (C.c_uint64, [U32P, C.c_uint64, C.c_uint32]),
+    "wk_gen_university": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(U32P), U64P]),
+    "wk_univ_id": (C.c_uint32, [C.c_char_p]),
+    "wk_gen_powerlaw": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                  C.c_double, C.c_uint64, C.POINTER(U32P), U64P]),
+    "wk_free": (None, [P]),
+    "wk_strerror": (C.c_char_p, [C.c_int]),
+    "wk_last_hip_error": (C.c_int, []),
+    "wk_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+}
+
+
+def lib():
+    """Load libwukong_gpu.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libwukong_gpu.so is not built (run __graft_entry__.build()): " + LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _strerror(code):
+    return lib().wk_strerror(code).decode()
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        raise WkError(rc, what)
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(U32P)
+
+
+def device_count():
+    n = C.c_int(0)
+    lib().wk_device_count(C.byref(n))
+    return n.value
+
+
+# ------------------------------------------------------------------ generators
+
+def univ_id(name):
+    return int(lib().wk_univ_id(name.encode()))
+
+
+def _take_triples(ptr, n):
+    if n == 0:
+        lib().wk_free(C.cast(ptr, P))
+        return np.zeros((0, 3), np.uint32)
+    arr = np.ctypeslib.as_array(ptr, shape=(n * 3,)).copy().reshape(n, 3)
+    lib().wk_free(C.cast(ptr, P))
+    return arr
+
+
+def gen_university(seed, n_univ):
+    """(n,3) uint32 triples of the seeded university-schema generator."""
+    ptr, n = U32P(), C.c_uint64(0)
+    _check(lib().wk_gen_university(seed, n_univ, C.byref(ptr), C.byref(n)), "wk_gen_university")
+    return _take_triples(ptr, n.value)
+
+
+def gen_powerlaw(seed, n_vertices, n_triples, n_preds=256, n_types=64, zipf_s=1.1, alpha=2.0,
+                 max_degree=10**6):
+    ptr, n = U32P(), C.c_uint64(0)
+    _check(lib().wk_gen_powerlaw(seed, n_vertices, n_triples, n_preds, n_types, zipf_s, alpha, max_degree,
+                                 C.byref(ptr), C.byref(n)), "wk_gen_powerlaw")
+    return _take_triples(ptr, n.value)
+
+
+def checksum(table):
+    t = np.ascontiguousarray(table, dtype=np.uint32)
+    if t.ndim == 1:
+        t = t.reshape(-1, 1)
+    rows, cols = t.shape
+    return int(lib().wk_table_checksum(t.ctypes.data_as(U32P), rows, cols))
+
+
+# ----------------------------------------------------------------------- store
+
+class Store:
+    """Host gstore (SegmentRDFGraph::init_gstore over pso/pos triples)."""
+
+    def __init__(self, pso, pos=None, pinned=False, load_factor_pct=0):
+        pso_a, pso_p = _u32(np.asarray(pso).reshape(-1))
+        if pos is None:
+            pos_a, pos_p = pso_a, pso_p
+        else:
+            pos_a, pos_p = _u32(np.asarray(pos).reshape(-1))
+        self._h = P()
+        opts = StoreOpts(1 if pinned else 0, load_factor_pct)
+        _check(lib().wk_store_build(pso_p, pso_a.size // 3, pos_p, pos_a.size // 3, C.byref(opts),
+                                    C.byref(self._h)), "wk_store_build")
+
+    def close(self):
+        if self._h:
+            lib().wk_store_destroy(self._h)
+            self._h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        i = StoreInfo()
+        _check(lib().wk_store_info_get(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in StoreInfo._fields_}
+
+    def segments(self):
+        out = []
+        for k in range(self.info()["num_segments"]):
+            s = SegInfo()
+            _check(lib().wk_store_segment(self._h, k, C.byref(s)))
+            out.append({f: getattr(s, f) for f, _ in SegInfo._fields_ if f != "pad"})
+        return out
+
+    def raw(self):
+        """(slots as (buckets*8, 2) uint64 view, values uint32 view) of the host arrays."""
+        sp, nb, vp, nv = P(), C.c_uint64(), U32P(), C.c_uint64()
+        _check(lib().wk_store_raw(self._h, C.byref(sp), C.byref(nb), C.byref(vp), C.byref(nv)))
+        slots = np.ctypeslib.as_array(C.cast(sp, U64P), shape=(nb.value * 8 * 2,)).reshape(-1, 2)
+        vals = np.ctypeslib.as_array(vp, shape=(nv.value,)) if nv.value else np.zeros(0, np.uint32)
+        return slots, vals
+
+    def get_triples(self, vid, pid, d):
+        p, n = U32P(), C.c_uint64()
+        _check(lib().wk_store_get_triples(self._h, vid, pid, d, C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+
+    def get_index(self, tpid, d):
+        p, n = U32P(), C.c_uint64()
+        _check(lib().wk_store_get_index(self._h, tpid, d, C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+
+
+def make_query(patterns, nvars):
+    """Pack [(s, p, o, dir), ...] into a wk_query (keeps the array alive on the object)."""
+    arr = (Pattern * max(1, len(patterns)))()
+    for i, (s, p, o, d) in enumerate(patterns):
+        arr[i] = Pattern(s, p, o, d)
+    q = Query(C.cast(arr, C.POINTER(Pattern)), len(patterns), nvars)
+    q._keep = arr
+    return q
+
+
+# ---------------------------------------------------------------------- device
+
+class Table:
+    def __init__(self, dev, h):
+        self.dev, self._h = dev, h
+
+    @property
+    def shape(self):
+        r, c = C.c_uint64(), C.c_uint32()
+        _check(lib().wk_table_shape(self._h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    def numpy(self):
+        r, c = self.shape
+        out = np.zeros((r, c), np.uint32)
+        _check(lib().wk_table_download(self.dev._h, self._h, out.ctypes.data_as(U32P), out.size),
+               "wk_table_download")
+        return out
+
+    def free(self):
+        if self._h:
+            lib().wk_table_free(self.dev._h, self._h)
+            self._h = P()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Device:
+    """HBM gstore cache + history tables + expansion kernels on one GPU."""
+
+    def __init__(self, store, device=0, gstore_budget=0, table_budget=0, key_block_shift=0,
+                 value_block_shift=0, enable_pipeline=1, probe_width=8, rows_per_wave=0, profile=0):
+        self.store = store
+        self._h = P()
+        o = DevOpts(device, gstore_budget, table_budget, key_block_shift, value_block_shift,
+                    enable_pipeline, probe_width, rows_per_wave, profile)
+        _check(lib().wk_dev_create(store.handle, C.byref(o), C.byref(self._h)), "wk_dev_create")
+
+    def close(self):
+        if self._h:
+            lib().wk_dev_destroy(self._h)
+            self._h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().wk_dev_sync(self._h))
+
+    def load_all(self):
+        _check(lib().wk_dev_load_all(self._h), "wk_dev_load_all")
+
+    def set_tuning(self, probe_width, rows_per_wave):
+        _check(lib().wk_dev_set_tuning(self._h, probe_width, rows_per_wave))
+
+    def stats(self):
+        s = DevStats()
+        _check(lib().wk_dev_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in DevStats._fields_}
+
+    def kernel_stats(self):
+        n = C.c_uint32()
+        _check(lib().wk_dev_kernel_stats(self._h, None, 0, C.byref(n)))
+        arr = (KernelStat * max(1, n.value))()
+        _check(lib().wk_dev_kernel_stats(self._h, arr, n.value, C.byref(n)))
+        return {arr[i].name.decode(): {"launches": arr[i].launches, "total_ms": arr[i].total_ms,
+                                       "alg_bytes": arr[i].alg_bytes} for i in range(n.value)}
+
+    def reset_kernel_stats(self):
+        _check(lib().wk_dev_reset_kernel_stats(self._h))
+
+    def seg_ensure(self, index, pid, d):
+        _check(lib().wk_seg_ensure(self._h, index, pid, d), "wk_seg_ensure")
+
+    def seg_resident(self, index, pid, d):
+        r = C.c_int32()
+        _check(lib().wk_seg_resident(self._h, index, pid, d, C.byref(r)))
+        return bool(r.value)
+
+    # per-pattern API
+    def upload(self, table):
+        t = np.ascontiguousarray(table, dtype=np.uint32)
+        if t.ndim == 1:
+            t = t.reshape(-1, 1)
+        h = P()
+        _check(lib().wk_table_upload(self._h, t.ctypes.data_as(U32P), t.shape[0], t.shape[1], C.byref(h)))
+        return Table(self, h)
+
+    def i2u(self, tpid, d):
+        h = P()
+        _check(lib().wk_start_i2u(self._h, tpid, d, C.byref(h)), "wk_start_i2u")
+        return Table(self, h)
+
+    def c2u(self, vid, pid, d):
+        h = P()
+        _check(lib().wk_start_c2u(self._h, vid, pid, d, C.byref(h)), "wk_start_c2u")
+        return Table(self, h)
+
+    def k2u(self, t, col, pid, d):
+        h = P()
+        _check(lib().wk_expand_k2u(self._h, t._h, col, pid, d, C.byref(h)), "wk_expand_k2u")
+        return Table(self, h)
+
+    def k2k(self, t, col, pid, d, end_col):
+        h = P()
+        _check(lib().wk_expand_k2k(self._h, t._h, col, pid, d, end_col, C.byref(h)), "wk_expand_k2k")
+        return Table(self, h)
+
+    def k2c(self, t, col, pid, d, end):
+        h = P()
+        _check(lib().wk_expand_k2c(self._h, t._h, col, pid, d, end, C.byref(h)), "wk_expand_k2c")
+        return Table(self, h)
+
+    # whole queries
+    def query(self, patterns, nvars, fetch=True):
+        """Per-query execution; returns (table (rows, cols) uint32 or None, v2c list, status)."""
+        q = make_query(patterns, nvars)
+        rows, cols = C.c_uint64(), C.c_uint32()
+        v2c = (C.c_int32 * max(1, nvars))()
+        rc = lib().wk_query_run(self._h, C.byref(q), C.byref(rows), C.byref(cols), v2c)
+        if rc not in (0,) and rc < 100:
+            return np.zeros((0, 0), np.uint32), [], rc
+        _check(rc, "wk_query_run")
+        out = None
+        if fetch:
+            out = np.zeros((rows.value, cols.value), np.uint32)
+            _check(lib().wk_result_download(self._h, out.ctypes.data_as(U32P), out.size))
+        return out, list(v2c[:nvars]), 0
+
+    def batch(self, queries, use_graph=True, fetch=True):
+        """queries: list of (patterns, nvars).  Returns list of (table, v2c, status) or, without
+        fetch, (rows, cols, status) arrays."""
+        qs = (Query * max(1, len(queries)))()
+        keep = []
+        for i, (pats, nv) in enumerate(queries):
+            q = make_query(pats, nv)
+            keep.append(q)
+            qs[i] = q
+        n = len(queries)
+        rows = np.zeros(n, np.uint64)
+        cols = np.zeros(n, np.uint32)
+        status = np.zeros(n, np.int32)
+        offs = np.zeros(n, np.uint64)
+        v2c = np.zeros(n * MAX_VARS, np.int32)
+        res = BatchResult(rows.ctypes.data_as(U64P), cols.ctypes.data_as(U32P),
+                          status.ctypes.data_as(C.POINTER(C.c_int32)), offs.ctypes.data_as(U64P),
+                          v2c.ctypes.data_as(C.POINTER(C.c_int32)))
+        _check(lib().wk_batch_run(self._h, qs, n, 1 if use_graph else 0, C.byref(res)), "wk_batch_run")
+        if not fetch:
+            return rows, cols, status
+        used = C.c_uint64()
+        _check(lib().wk_batch_download(self._h, None, 0, C.byref(used)))
+        arena = np.zeros(max(1, used.value), np.uint32)
+        _check(lib().wk_batch_download(self._h, arena.ctypes.data_as(U32P), arena.size, C.byref(used)))
+        out = []
+        for i in range(n):
+            r, c, o = int(rows[i]), int(cols[i]), int(offs[i])
+            t = arena[o:o + r * c].reshape(r, c) if c else np.zeros((0, 0), np.uint32)
+            out.append((t.copy(), list(v2c[i * MAX_VARS:i * MAX_VARS + queries[i][1]]), int(status[i])))
+        return out
+
+    def serve(self, queries, arrivals, max_inflight=4096, footprint_budget=0, per_query=False,
+              keep_results=False):
+        n = len(queries)
+        qs = (Query * max(1, n))()
+        keep = []
+        for i, (pats, nv) in enumerate(queries):
+            q = make_query(pats, nv)
+            keep.append(q)
+            qs[i] = q
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        lat = np.zeros(n, np.float64)
+        start = np.zeros(n, np.float64)
+        rows = np.zeros(n, np.uint64)
+        status = np.zeros(n, np.int32)
+        ck = np.zeros(n, np.uint64)
+        res = ServeResult(lat.ctypes.data_as(C.POINTER(C.c_double)), start.ctypes.data_as(C.POINTER(C.c_double)),
+                          rows.ctypes.data_as(U64P), status.ctypes.data_as(C.POINTER(C.c_int32)),
+                          ck.ctypes.data_as(U64P), 0.0, 0)
+        o = ServeOpts(max_inflight, footprint_budget, 1 if per_query else 0, 1 if keep_results else 0)
+        _check(lib().wk_serve_replay(self._h, qs, arr.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(o),
+                                     C.byref(res)), "wk_serve_replay")
+        return {"latency_us": lat, "start_us": start, "rows": rows, "status": status, "checksum": ck,
+                "wall_s": res.wall_s, "batches": res.batches}
+
+    def served_rows(self, i):
+        r, c = C.c_uint64(), C.c_uint32()
+        _check(lib().wk_serve_result_rows(self._h, i, None, 0, C.byref(r), C.byref(c)))
+        out = np.zeros((r.value, c.value), np.uint32)
+        _check(lib().wk_serve_result_rows(self._h, i, out.ctypes.data_as(U32P), out.size, C.byref(r),
+                                          C.byref(c)))
+        return out
