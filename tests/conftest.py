@@ -28,6 +28,14 @@ def _ensure_built():
 _ensure_built()
 
 
+@pytest.fixture(autouse=True)
+def _announce_gpu_test(request):
+    """Name each GPU test on stdout before it touches the device."""
+    if request.node.get_closest_marker("gpu"):
+        print("\n[gpu-test] start %s" % request.node.nodeid, flush=True)
+    yield
+
+
 @pytest.fixture(scope="session")
 def univ10():
     """Config-1 graph: university generator, seed 1, 10 universities."""

@@ -337,7 +337,7 @@ class Device:
     """HBM gstore cache + history tables + expansion kernels on one GPU."""
 
     def __init__(self, store, device=0, gstore_budget=0, table_budget=0, key_block_shift=0,
-                 value_block_shift=0, enable_pipeline=1, probe_width=8, rows_per_wave=0, profile=0):
+                 value_block_shift=0, enable_pipeline=1, probe_width=1, rows_per_wave=0, profile=0):
         self.store = store
         self._h = P()
         o = DevOpts(device, gstore_budget, table_budget, key_block_shift, value_block_shift,
