@@ -31,13 +31,7 @@ def test_config1_chain(univ10_dev, univ10_oracle):
     assert t.shape[0] > 20000 and t.shape[1] == 3
 
 
-LUBM_Q7_SKIP = ("the first GPU run faulted the device while this query ran on the 10-university "
-                 "graph; no defect has been shown yet (CPU emulation of the same kernels is clean), "
-                 "so the case stays off the GPU until a cause is found")
-
-
-@pytest.mark.parametrize("name", [pytest.param("lubm_q7", marks=pytest.mark.skip(reason=LUBM_Q7_SKIP)),
-                                  "type_start2", "lubm_q1"])
+@pytest.mark.parametrize("name", ["lubm_q7", "type_start2", "lubm_q1"])
 def test_lubm_shapes(univ10_dev, univ10_oracle, name):
     pats, nv = getattr(Q, name)()
     t = check_query(univ10_dev, univ10_oracle, pats, nv)

@@ -83,11 +83,11 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     unpin_segments(dev, segs);
     return code;
   };
-  if (hipMemcpyAsync(dev->d_desc, dev->h_desc, (S + 1) * qc * sizeof(StepDesc), hipMemcpyHostToDevice,
-                     dev->stream) ||
-      hipMemcpyAsync(dev->d_rs, dev->h_rs, (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream) ||
-      hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState), hipMemcpyHostToDevice,
-                     dev->stream))
+  if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, (S + 1) * qc * sizeof(StepDesc), hipMemcpyHostToDevice,
+                     dev->stream)) ||
+      hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
+      hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState), hipMemcpyHostToDevice,
+                     dev->stream)))
     return fail(WK_E_HIP);
 
   if (use_graph) {
@@ -96,23 +96,23 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     if (it == dev->graphs.end()) {
       hipGraph_t g = nullptr;
       hipGraphExec_t ge = nullptr;
-      if (hipStreamBeginCapture(dev->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return fail(WK_E_HIP);
+      if (hip_failed(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeRelaxed))) return fail(WK_E_HIP);
       launch_batch_steps(dev, S, nq, false);
-      if (hipStreamEndCapture(dev->stream, &g) != hipSuccess) return fail(WK_E_HIP);
-      if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+      if (hip_failed(hipStreamEndCapture(dev->stream, &g))) return fail(WK_E_HIP);
+      if (hip_failed(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0))) {
         (void)hipGraphDestroy(g);
         return fail(WK_E_HIP);
       }
       (void)hipGraphDestroy(g);
       it = dev->graphs.emplace(key, ge).first;
     }
-    if (hipGraphLaunch(it->second, dev->stream) != hipSuccess) return fail(WK_E_HIP);
+    if (hip_failed(hipGraphLaunch(it->second, dev->stream))) return fail(WK_E_HIP);
   } else {
     launch_batch_steps(dev, S, nq, dev->opts.profile != 0);
   }
-  if (hipMemcpyAsync(dev->h_desc, dev->d_desc, S * qc * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream) ||
-      hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream) ||
-      hipStreamSynchronize(dev->stream))
+  if (hip_failed(hipMemcpyAsync(dev->h_desc, dev->d_desc, S * qc * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream)) ||
+      hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
+      hip_failed(hipStreamSynchronize(dev->stream)))
     return fail(WK_E_HIP);
   unpin_segments(dev, segs);
   bool overflow = false;

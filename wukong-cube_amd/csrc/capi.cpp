@@ -361,7 +361,7 @@ int wk_table_upload(wk_dev* dev, const uint32_t* host, uint64_t rows, uint32_t c
       delete t;
       return WK_E_NOMEM_BUDGET;
     }
-    if (hipMemcpy(t->d, host, rows * cols * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hip_failed(hipMemcpy(t->d, host, rows * cols * sizeof(uint32_t), hipMemcpyHostToDevice))) {
       (void)hipFree(t->d);
       delete t;
       return WK_E_HIP;

@@ -379,9 +379,9 @@ int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** to
     unpin_segments(dev, segs);
     return code;
   };
-  if (hipMemcpyAsync(dev->d_desc, dev->h_desc, sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream) ||
-      hipMemcpyAsync(dev->d_rs, dev->h_rs, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream) ||
-      hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream))
+  if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream)) ||
+      hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
+      hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream)))
     return fail(WK_E_HIP);
   StepArgs a = make_args(dev, 0, 1, tin, alloc_out ? nullptr : *tout, alloc_out ? ~uint64_t(0) : tcap, false);
   const double R = double(sd.in_rows), C = double(sd.in_cols);
@@ -392,8 +392,8 @@ int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** to
   timed_begin(dev, &ev);
   launch_scan(a, sd.in_rows, dev->stream);
   timed_end(dev, ev, "scan", 4 * R + 8 * R);
-  if (hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream) ||
-      hipStreamSynchronize(dev->stream))
+  if (hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
+      hip_failed(hipStreamSynchronize(dev->stream)))
     return fail(WK_E_HIP);
   const StepState stt = dev->h_state[0];
   if (stt.status) return fail(WK_E_NOMEM_BUDGET);

@@ -16,6 +16,13 @@ namespace wk {
 
 extern thread_local int g_last_hip_error;
 
+// True when `e` is a failure; the code is kept for wk_last_hip_error().
+inline bool hip_failed(hipError_t e) {
+  if (e == hipSuccess) return false;
+  g_last_hip_error = int(e);
+  return true;
+}
+
 #define WK_HIP(x)                            \
   do {                                       \
     hipError_t e_ = (x);                     \

@@ -149,6 +149,36 @@ __device__ __forceinline__ uint64_t step_rows(const StepArgs& a) {
   return r < a.rcap ? r : a.rcap;
 }
 
+// The StepDesc fields the probe needs, read as three 16-byte words so that
+// every later choice between them is a choice between register values.
+// hipcc (ROCm 7.2) miscompiles the form `op == FILTER_KNOWN ? row[d.ec] :
+// d.kend` inside the probe: structurizecfg turns the selected pointer into
+// poison on the known_to_const path, which then loads from a stale register.
+struct ProbeDesc {
+  uint32_t op, pid, dir, key_mode, sc, ec, kstart, kend, in_cols;
+  int32_t seg;
+};
+static_assert(offsetof(StepDesc, key_mode) == 16 && offsetof(StepDesc, kend) == 32 &&
+                  offsetof(StepDesc, in_cols) == 36 && sizeof(StepDesc) % 16 == 0,
+              "StepDesc word layout");
+
+__device__ __forceinline__ ProbeDesc read_probe_desc(const StepDesc* d) {
+  const uint4* w = reinterpret_cast<const uint4*>(d);
+  const uint4 w0 = w[0], w1 = w[1], w2 = w[2];
+  ProbeDesc p;
+  p.op = w0.x;
+  p.seg = int32_t(w0.y);
+  p.pid = w0.z;
+  p.dir = w0.w;
+  p.key_mode = w1.x;
+  p.sc = w1.y;
+  p.ec = w1.z;
+  p.kstart = w1.w;
+  p.kend = w2.x;
+  p.in_cols = w2.y;
+  return p;
+}
+
 // ---- probe: per input row, edge count (expand) or keep flag (filters) -----
 
 template <int W>
@@ -165,22 +195,16 @@ __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
       if (g < R) {
         uint32_t c = 0, eo = 0;
         const uint32_t q = find_query(a.row_start, a.nq, g);
-        const StepDesc& d = a.desc[q];
+        const ProbeDesc d = read_probe_desc(a.desc + q);
         if (d.op != OP_SKIP && d.seg >= 0) {
           const DevSeg s = a.segs[d.seg];
-          const uint32_t* row = a.tin + d.in_off + (g - a.row_start[q]) * d.in_cols;
-          uint64_t key = 0;
-          bool valid = s.resident != 0;
-          if (d.key_mode == KEY_CONST) {
-            key = make_key(d.kstart, d.pid, d.dir);
-          } else if (d.key_mode == KEY_TYPE_INDEX) {
-            // [0|t|IN]: only type/predicate ids fit the 17-bit pid field
-            const uint32_t v = row[d.sc];
-            valid = valid && v < (1u << NBITS_IDX);
-            key = make_key(0, v, DIR_IN);
-          } else {
-            key = make_key(row[d.sc], d.pid, d.dir);
-          }
+          const uint32_t* row = a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols;
+          // the key's vertex: a column of the row, or the constant start
+          const uint32_t kv = (d.key_mode == KEY_CONST) ? d.kstart : row[d.sc];
+          // KEY_TYPE_INDEX reads [0|t|IN]: only type/predicate ids fit the 17-bit pid field
+          const bool tix = d.key_mode == KEY_TYPE_INDEX;
+          const uint64_t key = tix ? make_key(0, kv, DIR_IN) : make_key(kv, d.pid, d.dir);
+          const bool valid = s.resident != 0 && (!tix || kv < (1u << NBITS_IDX));
           uint64_t ptr = 0;
           if (valid && probe<W>(a, s, key, ptr)) {
             const uint64_t sz = ptr_size(ptr);
@@ -189,7 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
               c = uint32_t(sz);
               eo = uint32_t(off);
             } else {
-              const uint32_t target = (d.op == OP_FILTER_KNOWN) ? row[d.ec] : d.kend;
+              uint32_t target = d.kend;
+              if (d.op == OP_FILTER_KNOWN) target = row[d.ec];
               c = contains(a, s, off, sz, target) ? 1u : 0u;
             }
           }

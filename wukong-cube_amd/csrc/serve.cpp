@@ -126,11 +126,11 @@ struct Server {
       if (rc) return rc;
       const uint32_t* tin = dev->tbuf[cur];
       uint32_t* tout = dev->tbuf[cur ^ 1];
-      if (hipMemcpyAsync(dev->d_desc, dev->h_desc, n * sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream) ||
-          hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream) ||
-          hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream) ||
-          hipMemcpyAsync(d_idx, h_idx, std::max<uint32_t>(nfinal, 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
-                         dev->stream)) {
+      if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, n * sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream)) ||
+          hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
+          hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream)) ||
+          hip_failed(hipMemcpyAsync(d_idx, h_idx, std::max<uint32_t>(nfinal, 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
       }
@@ -141,14 +141,14 @@ struct Server {
       if (checksum_now && nfinal) {
         launch_fill_u64(d_sums, nfinal, 0, dev->stream);
         launch_checksum_final(dev->arena, dev->d_desc, d_idx, nfinal, dev->d_state, d_sums, dev->stream);
-        if (hipMemcpyAsync(h_sums, d_sums, nfinal * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream)) {
+        if (hip_failed(hipMemcpyAsync(h_sums, d_sums, nfinal * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
           unpin_segments(dev, segs);
           return WK_E_HIP;
         }
       }
-      if (hipMemcpyAsync(dev->h_desc, dev->d_desc, n * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream) ||
-          hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream) ||
-          hipStreamSynchronize(dev->stream)) {
+      if (hip_failed(hipMemcpyAsync(dev->h_desc, dev->d_desc, n * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream)) ||
+          hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
+          hip_failed(hipStreamSynchronize(dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
       }
@@ -272,14 +272,14 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
       uint64_t ck = 0;
       if (st == WK_OK && rows) {
         CkDesc c{0, rows, 0, plans[q].cols, 0};
-        if (hipMemcpyAsync(d_ck, &c, sizeof(c), hipMemcpyHostToDevice, dev->stream)) {
+        if (hip_failed(hipMemcpyAsync(d_ck, &c, sizeof(c), hipMemcpyHostToDevice, dev->stream))) {
           rc = WK_E_HIP;
           break;
         }
         launch_fill_u64(d_sum, 1, 0, dev->stream);
         launch_checksum(dev->tbuf[buf], d_ck, 1, rows, d_sum, dev->stream);
-        if (hipMemcpyAsync(&ck, d_sum, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream) ||
-            hipStreamSynchronize(dev->stream)) {
+        if (hip_failed(hipMemcpyAsync(&ck, d_sum, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream)) ||
+            hip_failed(hipStreamSynchronize(dev->stream))) {
           rc = WK_E_HIP;
           break;
         }
