@@ -38,6 +38,19 @@ def test_lubm_shapes(univ10_dev, univ10_oracle, name):
     assert t.shape[0] > 0
 
 
+def test_config1_fixtures(univ10_dev):
+    """Device results of the committed config-1 fixtures: row count, var->col
+    map and order-independent checksum."""
+    import json
+    import os
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1.json")))
+    for name, want in golden["queries"].items():
+        got, v2c, rc = univ10_dev.query([tuple(p) for p in want["patterns"]], want["nvars"])
+        assert rc == want["status"], name
+        assert got.shape[0] == want["rows"] and list(v2c) == want["v2c"], name
+        assert str(wg.checksum(got) if got.size else 0) == want["checksum"], name
+
+
 def test_per_pattern_api(univ10_dev, univ10_oracle):
     """i2u -> k2u -> k2c -> k2k through the per-pattern entry points."""
     dev, orc = univ10_dev, univ10_oracle

@@ -288,7 +288,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {
 }
 
 __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
-  if (a.st->status) return;
+  if (a.st->status) {
+    // an overflowed step skips everything after it: pass the status on so
+    // the next step's kernels also return before reading its (unwritten) rows
+    if (threadIdx.x == 0 && a.next_st) a.next_st->status = a.st->status;
+    return;
+  }
   const uint64_t R = step_rows(a);
   uint64_t carry_t = a.st->tout_base, carry_a = a.st->arena_base, carry_n = 0;
   for (uint32_t base = 0; base < a.nq; base += 1024) {
