@@ -143,16 +143,40 @@ def test_batch_mixed_random(univ10_oracle):
 
 
 def test_batch_overflow_splits(univ10_store, univ10_oracle):
-    """A table budget far below the batch's total output forces splitting;
-    a query that alone exceeds it reports WK_E_NOMEM_BUDGET."""
-    dev = wg.Device(univ10_store, table_budget=4 << 20)   # ~150K elements per table
-    stream = Q.template_stream(univ10_store, Q.univ_templates(), 200, seed=9)
-    qs = [(p, nv) for p, nv, _ in stream] + [Q.type_start2()]
+    """A batch whose step table exceeds the table capacity is split and still
+    exact; a query that alone exceeds it reports WK_E_NOMEM_BUDGET."""
+    dev = wg.Device(univ10_store, table_budget=4 << 20)   # 149,796 elements per table
+    tcap = dev.stats()["table_capacity_elems"]
+    GS, MEM = Q.U("ub:GraduateStudent"), Q.U("ub:memberOf")
+    depts = np.sort(univ10_store.get_index(Q.U("ub:Department"), wg.IN))[:8]
+    # index start (23,151 rows each) then a known_to_const filter on one department
+    qs = [([(GS, wg.TYPE_ID, -1, wg.IN), (-1, MEM, int(d), wg.OUT)], 1) for d in depts]
+    assert 23151 * len(qs) > tcap  # step 0 of the whole batch cannot fit: the batch must split
+    qs.append(Q.type_start2())     # 220,425 x 2 elements alone: cannot fit
     res = dev.batch(qs, use_graph=False)
     for (pats, nv), (t, v2c, rc) in zip(qs[:-1], res[:-1]):
-        want, _, _ = univ10_oracle.query(pats, nv)
-        assert rc == 0 and same_rows(t, want)
+        want, _, wrc = univ10_oracle.query(pats, nv)
+        assert wrc == 0 and rc == 0 and want.shape[0] > 0
+        assert same_rows(t, want), pats
     assert res[-1][2] == wg.WK_E_NOMEM_BUDGET
+    dev.close()
+
+
+def test_batch_arena_exhaustion(univ10_store, univ10_oracle):
+    """Results kept in the batch arena beyond its capacity: every query is
+    either exact or reports WK_E_NOMEM_BUDGET, never wrong rows."""
+    dev = wg.Device(univ10_store, table_budget=4 << 20)
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 200, seed=9)
+    qs = [(p, nv) for p, nv, _ in stream]
+    res = dev.batch(qs, use_graph=False)
+    ok = 0
+    for (pats, nv), (t, v2c, rc) in zip(qs, res):
+        assert rc in (0, wg.WK_E_NOMEM_BUDGET), rc
+        if rc == 0:
+            want, _, _ = univ10_oracle.query(pats, nv)
+            assert same_rows(t, want), pats
+            ok += 1
+    assert ok >= 50
     dev.close()
 
 
