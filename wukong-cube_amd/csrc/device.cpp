@@ -4,6 +4,7 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace wk {
@@ -386,12 +387,22 @@ int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** to
   StepArgs a = make_args(dev, 0, 1, tin, alloc_out ? nullptr : *tout, alloc_out ? ~uint64_t(0) : tcap, false);
   const double R = double(sd.in_rows), C = double(sd.in_cols);
   hipEvent_t ev;
+  // per-kernel HIP-event timing (opts.profile) with the algorithmic bytes of
+  // each launch: key column 4 B + slot 16 B + cnt/eoff 8 B per row for the
+  // probe, cnt 4 B + prefix 8 B per row for the row scan
+  const double ntiles = std::ceil(R / double(kTile));
   timed_begin(dev, &ev);
   launch_probe(a, dev->opts.probe_width, sd.in_rows, dev->stream);
-  timed_end(dev, ev, sd.op == OP_EXPAND ? "probe_expand" : "probe_filter", 4 * R + 16 * R + 8 * R);
+  timed_end(dev, ev, "k_probe", 28 * R);
   timed_begin(dev, &ev);
-  launch_scan(a, sd.in_rows, dev->stream);
-  timed_end(dev, ev, "scan", 4 * R + 8 * R);
+  launch_scan_tiles(a, dev->stream);
+  timed_end(dev, ev, "k_scan_tiles", 16 * ntiles);
+  timed_begin(dev, &ev);
+  launch_scan_rows(a, sd.in_rows, dev->stream);
+  timed_end(dev, ev, "k_scan_rows", 12 * R);
+  timed_begin(dev, &ev);
+  launch_scan_queries(a, dev->stream);
+  timed_end(dev, ev, "k_scan_queries", 96);
   if (hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
       hip_failed(hipStreamSynchronize(dev->stream)))
     return fail(WK_E_HIP);
@@ -410,9 +421,14 @@ int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** to
   if (total) {
     const double E = double(total), Co = double(sd.out_cols);
     const double bytes = sd.op == OP_EXPAND ? 4 * R * C + 4 * E + 4 * E * Co : 4 * R * C + 4 * E * Co;
+    if (dev->opts.rows_per_wave <= 0) {
+      timed_begin(dev, &ev);
+      launch_tile_rows(a, total, dev->stream);
+      timed_end(dev, ev, "k_tile_rows", 8 * std::ceil(E / kOutTile));
+    }
     timed_begin(dev, &ev);
-    launch_gather(a, dev->opts.rows_per_wave, total, dev->stream);
-    timed_end(dev, ev, sd.op == OP_EXPAND ? "gather_expand" : "gather_filter", bytes);
+    launch_gather_only(a, dev->opts.rows_per_wave, total, dev->stream);
+    timed_end(dev, ev, dev->opts.rows_per_wave <= 0 ? "k_gather" : "k_gather_rows", bytes);
   }
   unpin_segments(dev, segs);
   *out_rows = total;

@@ -275,15 +275,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {
       const uint64_t g = g0 + j;
       if (g < R) {
         a.pfx[g] = ex;
-        if (c[j]) {
-          // this row holds the first output row of every tile starting in [ex, ex+c)
-          uint64_t tt = (ex + kOutTile - 1) / kOutTile;
-          const uint64_t last = (ex + c[j] - 1) / kOutTile;
-          for (; tt <= last && tt < a.otile_cap; ++tt) a.tile_row[tt] = g;
-        }
         ex += c[j];
       }
     }
+  }
+}
+
+// Output tile -> source row of its first output row: the largest g with
+// pfx[g] <= t * kOutTile (a zero-count row can never be that largest g, as
+// the row after it has the same prefix).  One thread per output tile, so a
+// single row with millions of edges costs nothing extra.
+__global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  uint64_t nt = (a.st->total_out_rows + kOutTile - 1) / kOutTile;
+  if (nt > a.otile_cap) nt = a.otile_cap;
+  for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < nt; t += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t j = t * kOutTile;
+    uint64_t lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (a.pfx[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    a.tile_row[t] = lo;
   }
 }
 
@@ -472,16 +487,35 @@ void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
   hipLaunchKernelGGL(k_probe<1>, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
-void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
+void launch_scan_tiles(const StepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, a);
+}
+
+void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
   const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
   hipLaunchKernelGGL(k_scan_rows, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_scan_queries(const StepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
 }
 
-void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
+void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
+  launch_scan_tiles(a, s);
+  launch_scan_rows(a, rows_hint, s);
+  launch_scan_queries(a, s);
+}
+
+void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s) {
+  const uint64_t tiles = (out_hint + kOutTile - 1) / kOutTile;
+  const unsigned grid = out_hint ? grid_for((tiles + kBlock - 1) / kBlock, 1u << 20) : kPersistentGrid;
+  hipLaunchKernelGGL(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
   if (rows_per_wave <= 0) {
-    const unsigned grid = out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : 4 * kPersistentGrid;
+    const unsigned grid =
+        out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : 4 * kPersistentGrid;
     hipLaunchKernelGGL(k_gather, dim3(grid), dim3(kBlock), 0, s, a);
     return;
   }
@@ -493,6 +527,11 @@ void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipS
     case 4: hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(kBlock), 0, s, a); break;
     default: hipLaunchKernelGGL(k_gather_rows<64>, dim3(grid), dim3(kBlock), 0, s, a); break;
   }
+}
+
+void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
+  if (rows_per_wave <= 0) launch_tile_rows(a, out_hint, s);
+  launch_gather_only(a, rows_per_wave, out_hint, s);
 }
 
 void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t total_rows, uint64_t* sums,

@@ -100,8 +100,15 @@ constexpr int kOutTile = kBlock;                // output rows per gather tile
 // re-read the device sizes and stride).  `rows_hint`/`out_hint` = 0 means
 // unknown on the host: a fixed persistent grid is used.
 void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s);
+// scan = scan_tiles + scan_rows + scan_queries
 void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s);
+void launch_scan_tiles(const StepArgs& a, hipStream_t s);
+void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s);
+void launch_scan_queries(const StepArgs& a, hipStream_t s);
+// gather = tile_rows (output-balanced mapping only) + the gather kernel
 void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
+void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s);
+void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 
 // One non-empty table for the checksum kernel: `rows` rows of `cols`

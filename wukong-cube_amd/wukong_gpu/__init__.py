@@ -333,6 +333,30 @@ class Table:
             pass
 
 
+class PreparedBatch:
+    """A batch of queries packed once into wk_query structs, with its result
+    arrays: repeated Device.batch calls then cost only the C call."""
+
+    def __init__(self, queries):
+        self.n = len(queries)
+        self.qs = (Query * max(1, self.n))()
+        self._keep = []
+        self.nvars = []
+        for i, (pats, nv) in enumerate(queries):
+            q = make_query(pats, nv)
+            self._keep.append(q)
+            self.qs[i] = q
+            self.nvars.append(nv)
+        self.rows = np.zeros(self.n, np.uint64)
+        self.cols = np.zeros(self.n, np.uint32)
+        self.status = np.zeros(self.n, np.int32)
+        self.offs = np.zeros(self.n, np.uint64)
+        self.v2c = np.zeros(self.n * MAX_VARS, np.int32)
+        self.res = BatchResult(self.rows.ctypes.data_as(U64P), self.cols.ctypes.data_as(U32P),
+                               self.status.ctypes.data_as(C.POINTER(C.c_int32)), self.offs.ctypes.data_as(U64P),
+                               self.v2c.ctypes.data_as(C.POINTER(C.c_int32)))
+
+
 class Device:
     """HBM gstore cache + history tables + expansion kernels on one GPU."""
 
@@ -439,35 +463,22 @@ class Device:
         return out, list(v2c[:nvars]), 0
 
     def batch(self, queries, use_graph=True, fetch=True):
-        """queries: list of (patterns, nvars).  Returns list of (table, v2c, status) or, without
-        fetch, (rows, cols, status) arrays."""
-        qs = (Query * max(1, len(queries)))()
-        keep = []
-        for i, (pats, nv) in enumerate(queries):
-            q = make_query(pats, nv)
-            keep.append(q)
-            qs[i] = q
-        n = len(queries)
-        rows = np.zeros(n, np.uint64)
-        cols = np.zeros(n, np.uint32)
-        status = np.zeros(n, np.int32)
-        offs = np.zeros(n, np.uint64)
-        v2c = np.zeros(n * MAX_VARS, np.int32)
-        res = BatchResult(rows.ctypes.data_as(U64P), cols.ctypes.data_as(U32P),
-                          status.ctypes.data_as(C.POINTER(C.c_int32)), offs.ctypes.data_as(U64P),
-                          v2c.ctypes.data_as(C.POINTER(C.c_int32)))
-        _check(lib().wk_batch_run(self._h, qs, n, 1 if use_graph else 0, C.byref(res)), "wk_batch_run")
+        """queries: list of (patterns, nvars) or a PreparedBatch.  Returns list of
+        (table, v2c, status) or, without fetch, (rows, cols, status) arrays."""
+        pb = queries if isinstance(queries, PreparedBatch) else PreparedBatch(queries)
+        rows, cols, status, offs, v2c = pb.rows, pb.cols, pb.status, pb.offs, pb.v2c
+        _check(lib().wk_batch_run(self._h, pb.qs, pb.n, 1 if use_graph else 0, C.byref(pb.res)), "wk_batch_run")
         if not fetch:
-            return rows, cols, status
+            return rows.copy(), cols.copy(), status.copy()
         used = C.c_uint64()
         _check(lib().wk_batch_download(self._h, None, 0, C.byref(used)))
         arena = np.zeros(max(1, used.value), np.uint32)
         _check(lib().wk_batch_download(self._h, arena.ctypes.data_as(U32P), arena.size, C.byref(used)))
         out = []
-        for i in range(n):
+        for i in range(pb.n):
             r, c, o = int(rows[i]), int(cols[i]), int(offs[i])
             t = arena[o:o + r * c].reshape(r, c) if c else np.zeros((0, 0), np.uint32)
-            out.append((t.copy(), list(v2c[i * MAX_VARS:i * MAX_VARS + queries[i][1]]), int(status[i])))
+            out.append((t.copy(), list(v2c[i * MAX_VARS:i * MAX_VARS + pb.nvars[i]]), int(status[i])))
         return out
 
     def serve(self, queries, arrivals, max_inflight=4096, footprint_budget=0, per_query=False,
