@@ -36,6 +36,21 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` as measured by the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes summarised under profiles/ (a
+    `traffic: kernel=... workload=... bytes_per_launch=...` line), or None
+    when no measurement of this kernel and workload is there."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*hbm_traffic*.md"))):
+        for line in open(path):
+            m = re.match(r"traffic: kernel=(\S+) workload=(\S+) bytes_per_launch=(\d+)", line.strip())
+            if m and m.group(1) == kernel and m.group(2) == workload:
+                return int(m.group(3))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,8 +130,9 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     edges = sum(r["rows"] for r in single.values())
 
+    traffic = measured_traffic(dom, "univ%d-single" % args.univ)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": d["launches"]}
     if args.single_only:
         for r in single.values():
