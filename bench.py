@@ -149,7 +149,7 @@ def main():
     pb = wg.PreparedBatch(qs)  # packed once: the timed calls are the C batch entry point
     if args.tune:
         best = None
-        for pw in (1,):  # cooperative probe widths are switched off
+        for pw in (1, 2, 4, 8):
             for rpw in (0, 1, 4, 16, 64):
                 dev.set_tuning(pw, rpw)
                 dev.batch(pb, use_graph=not args.no_graph, fetch=False)

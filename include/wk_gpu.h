@@ -150,8 +150,7 @@ typedef struct {
   uint32_t key_block_shift;   /* log2 buckets per key block; 0 -> 14 (2 MiB) */
   uint32_t value_block_shift; /* log2 edges per value block; 0 -> 19 (2 MiB) */
   int32_t enable_pipeline;    /* prefetch next pattern's segment (global.hpp:136) */
-  int32_t probe_width;        /* lanes per key in the probe: 1 (0 -> 1); the
-                                 cooperative widths 2,4,8 are switched off */
+  int32_t probe_width;        /* lanes sharing one key's bucket probe: 1,2,4,8 (0 -> 1) */
   int32_t rows_per_wave;      /* gather mapping: 1,4,16,64 or 0 = balanced search */
   int32_t profile;            /* 1: time every kernel with HIP events */
 } wk_dev_opts;

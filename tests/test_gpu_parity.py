@@ -96,19 +96,20 @@ def test_random_graphs(seed):
     dev.close()
 
 
+@pytest.mark.parametrize("pw", [1, 2, 4, 8])
 @pytest.mark.parametrize("rpw", [0, 1, 4, 16, 64])
-def test_tuning_knobs(univ10_dev, univ10_oracle, rpw):
-    """Gather mappings (rows per wavefront); the probe runs one lane per key
-    (the cooperative widths are switched off and rejected)."""
-    univ10_dev.set_tuning(1, rpw)
+def test_tuning_knobs(univ10_dev, univ10_oracle, pw, rpw):
+    """Every probe width (lanes per key) and gather mapping (rows per
+    wavefront) gives the oracle's rows, filters included."""
+    univ10_dev.set_tuning(pw, rpw)
     try:
-        for f in (Q.chain3, Q.type_start2, Q.lubm_q1):
+        for f in (Q.chain3, Q.type_start2, Q.lubm_q1, Q.lubm_q7):
             pats, nv = f()
             check_query(univ10_dev, univ10_oracle, pats, nv)
     finally:
         univ10_dev.set_tuning(1, 0)
     with pytest.raises(wg.WkError):
-        univ10_dev.set_tuning(8, 0)
+        univ10_dev.set_tuning(3, 0)
 
 
 @pytest.mark.parametrize("use_graph", [False, True])

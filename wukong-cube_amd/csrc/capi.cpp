@@ -43,9 +43,9 @@ void free_dev(wk_dev* d) {
 int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   d->st = st;
   d->opts = *o;
-  // the wave-cooperative probe widths (2, 4, 8) are switched off: one lane per key
-  if (d->opts.probe_width != 0 && d->opts.probe_width != 1) return WK_E_INVALID_ARG;
-  d->opts.probe_width = 1;
+  if (d->opts.probe_width == 0) d->opts.probe_width = 1;
+  if (d->opts.probe_width != 1 && d->opts.probe_width != 2 && d->opts.probe_width != 4 && d->opts.probe_width != 8)
+    return WK_E_INVALID_ARG;
   if (d->opts.rows_per_wave != 1 && d->opts.rows_per_wave != 4 && d->opts.rows_per_wave != 16 &&
       d->opts.rows_per_wave != 64)
     d->opts.rows_per_wave = 0;
@@ -296,7 +296,7 @@ int wk_dev_reset_kernel_stats(wk_dev* dev) {
 
 int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
   if (!dev) return WK_E_INVALID_ARG;
-  if (probe_width != 1) return WK_E_INVALID_ARG;  // widths 2, 4, 8 are switched off
+  if (probe_width != 1 && probe_width != 2 && probe_width != 4 && probe_width != 8) return WK_E_INVALID_ARG;
   if (rows_per_wave != 0 && rows_per_wave != 1 && rows_per_wave != 4 && rows_per_wave != 16 &&
       rows_per_wave != 64)
     return WK_E_INVALID_ARG;

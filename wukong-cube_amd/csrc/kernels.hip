@@ -96,16 +96,13 @@ __device__ __forceinline__ bool probe(const StepArgs& a, const DevSeg& s, uint64
 #pragma unroll
     for (int k = 0; k < ASSOCIATIVITY / W; ++k) {
       const int si = gl + k * W;
-      const uint4 v = bk[si];
+      const uint4 v = bk[si];  // whole 16-B slot: key and ptr in one load
       const uint64_t sk = uint64_t(v.x) | (uint64_t(v.y) << 32);
-      if (si < ASSOCIATIVITY - 1) {
-        if (sk == key) {
-          hit = true;
-          hptr = uint64_t(v.z) | (uint64_t(v.w) << 32);
-        }
-      } else {
-        link = sk;
-      }
+      const uint64_t sp = uint64_t(v.z) | (uint64_t(v.w) << 32);
+      const bool m = si < ASSOCIATIVITY - 1 && sk == key;
+      hit = hit || m;
+      hptr = m ? sp : hptr;
+      link = (si == ASSOCIATIVITY - 1) ? sk : link;
     }
     if constexpr (W == 1) {
       if (hit) {
@@ -480,11 +477,14 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
 
 void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
   const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
-  // Only the one-lane-per-key probe is launched: the wave-cooperative
-  // variants (W = 2, 4, 8 lanes per key) are switched off while a GPU fault
-  // seen with them enabled is unexplained.
-  (void)w;
-  hipLaunchKernelGGL(k_probe<1>, dim3(grid), dim3(kBlock), 0, s, a);
+  // w lanes share one key: each reads 8/w slots of the 128-B bucket, so a
+  // wave's loads of one slot index cover 64/w buckets, each line coalesced
+  switch (w) {
+    case 8: hipLaunchKernelGGL(k_probe<8>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_probe<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_probe<2>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_probe<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
 }
 
 void launch_scan_tiles(const StepArgs& a, hipStream_t s) {
