@@ -62,7 +62,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--tune", action="store_true", help="sweep rows per wavefront of the gather")
+    ap.add_argument("--tune", action="store_true", help="sweep probe width x rows per wavefront on the batch")
+    ap.add_argument("--probe-width", type=int, default=1, help="lanes per key in the probe (1, 2, 4, 8)")
+    ap.add_argument("--rows-per-wave", type=int, default=0, help="gather mapping: 0 = output-balanced")
     ap.add_argument("--no-graph", action="store_true", help="launch batched steps directly (no hipGraph)")
     ap.add_argument("--single-only", action="store_true",
                     help="only the single-query kernels and their roofline (the launches a kernel trace "
@@ -85,7 +87,8 @@ def main():
     store = wg.Store(tri, pinned=True)
     info = store.info()
     log("host gstore built in %.1fs: %s" % (time.time() - t0, info))
-    dev = wg.Device(store, table_budget=48 << 30, profile=1)
+    dev = wg.Device(store, table_budget=48 << 30, profile=1, probe_width=args.probe_width,
+                    rows_per_wave=args.rows_per_wave)
     t0 = time.time()
     dev.load_all()
     st = dev.stats()
