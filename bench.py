@@ -62,8 +62,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--tune", action="store_true", help="sweep probe width x rows per wavefront on the batch")
-    ap.add_argument("--probe-width", type=int, default=1, help="lanes per key in the probe (1, 2, 4, 8)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the native autotuner (probe width x gather mapping) before the batch")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched stream (for kernel traces)")
+    ap.add_argument("--probe-width", type=int, default=0, help="lanes per key in the probe (1, 2, 4, 8; 0 = library default)")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="gather mapping: 0 = output-balanced")
     ap.add_argument("--no-graph", action="store_true", help="launch batched steps directly (no hipGraph)")
     ap.add_argument("--single-only", action="store_true",
@@ -98,7 +100,7 @@ def main():
 
     # ---------------------------------------------------------------- single
     single = {}
-    for name in ("lubm_q1", "type_start2"):
+    for name in (() if args.batch_only else ("lubm_q1", "type_start2")):
         pats, nv = getattr(Q, name)()
         for _ in range(args.warmup):
             dev.query(pats, nv, fetch=False)
@@ -120,23 +122,24 @@ def main():
         log("%s: %d rows, %.0f us/query" % (name, rows, np.mean(lat)))
 
     # roofline of the dominant kernel over both single queries
+    roofline = None
+    edges = sum(r["rows"] for r in single.values())
     agg = {}
     for name, r in single.items():
         for k, v in r["kernels"].items():
             a = agg.setdefault(k, {"launches": 0, "total_ms": 0.0, "alg_bytes": 0.0})
             for f in a:
                 a[f] += v[f]
-    dom = max(agg, key=lambda k: agg[k]["total_ms"])
-    d = agg[dom]
-    avg_ms = d["total_ms"] / d["launches"]
-    bytes_per_launch = d["alg_bytes"] / d["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    edges = sum(r["rows"] for r in single.values())
-
-    traffic = measured_traffic(dom, "univ%d-single" % args.univ)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": d["launches"]}
+    if agg:
+        dom = max(agg, key=lambda k: agg[k]["total_ms"])
+        d = agg[dom]
+        avg_ms = d["total_ms"] / d["launches"]
+        bytes_per_launch = d["alg_bytes"] / d["launches"]
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(dom, "univ%d-single" % args.univ)
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                    "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": d["launches"]}
     if args.single_only:
         for r in single.values():
             for k in ("_table", "_pats", "_nv"):
@@ -150,22 +153,10 @@ def main():
     stream = Q.template_stream(store, Q.univ_templates(), args.nq, seed=args.seed + 100)
     qs = [(p, nv) for p, nv, _ in stream]
     pb = wg.PreparedBatch(qs)  # packed once: the timed calls are the C batch entry point
-    if args.tune:
-        best = None
-        for pw in (1, 2, 4, 8):
-            for rpw in (0, 1, 4, 16, 64):
-                dev.set_tuning(pw, rpw)
-                dev.batch(pb, use_graph=not args.no_graph, fetch=False)
-                dev.sync()
-                t = time.perf_counter()
-                for _ in range(3):
-                    dev.batch(pb, use_graph=not args.no_graph, fetch=False)
-                el = (time.perf_counter() - t) / 3
-                log("tune probe_width=%d rows_per_wave=%d: %.3f ms/batch" % (pw, rpw, el * 1e3))
-                if best is None or el < best[0]:
-                    best = (el, pw, rpw)
-        dev.set_tuning(best[1], best[2])
-        out["tuning"] = {"probe_width": best[1], "rows_per_wave": best[2]}
+    if not args.no_tune:
+        pw, rpw, tms = dev.autotune(pb, use_graph=not args.no_graph)
+        log("autotune: probe_width=%d rows_per_wave=%d (%.3f ms per batch in the sweep)" % (pw, rpw, tms))
+        out["tuning"] = {"probe_width": pw, "rows_per_wave": rpw, "sweep_ms_per_batch": tms}
     for _ in range(args.warmup):
         rows_b, cols_b, st_b = dev.batch(pb, use_graph=not args.no_graph, fetch=False)
     dev.sync()
@@ -177,6 +168,12 @@ def main():
     assert (st_b == 0).all()
     qps_batched = args.nq / batch_s
     log("batched: %.3f ms per %d queries -> %.0f q/s" % (batch_s * 1e3, args.nq, qps_batched))
+    if args.batch_only:
+        print(json.dumps({"metric": "batched queries/s", "value": qps_batched, "ms_per_step": batch_s * 1e3,
+                          "tuning": out.get("tuning"), "config": {"universities": args.univ,
+                                                                   "queries_per_step": args.nq}}))
+        dev.close()
+        return
     # per-query execution of the same stream (bounded sample)
     npq = min(args.nq, 1024)
     t = time.perf_counter()

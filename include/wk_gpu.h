@@ -150,7 +150,7 @@ typedef struct {
   uint32_t key_block_shift;   /* log2 buckets per key block; 0 -> 14 (2 MiB) */
   uint32_t value_block_shift; /* log2 edges per value block; 0 -> 19 (2 MiB) */
   int32_t enable_pipeline;    /* prefetch next pattern's segment (global.hpp:136) */
-  int32_t probe_width;        /* lanes sharing one key's bucket probe: 1,2,4,8 (0 -> 1) */
+  int32_t probe_width;        /* lanes sharing one key's bucket probe: 1,2,4,8 (0 -> 2) */
   int32_t rows_per_wave;      /* gather mapping: 1,4,16,64 or 0 = balanced search */
   int32_t profile;            /* 1: time every kernel with HIP events */
 } wk_dev_opts;
@@ -235,6 +235,11 @@ typedef struct {
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph,
                  wk_batch_result* out);
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems);
+/* Autotuner (no counterpart in the reference): times the batch of nq
+ * queries under every probe width {1,2,4,8} x gather mapping {0,1,4,16,64}
+ * and keeps the fastest on the handle.  Outputs may be null. */
+int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, int32_t* probe_width,
+                    int32_t* rows_per_wave, double* batch_ms);
 
 /* ---------------------------------------------------------------- serving */
 

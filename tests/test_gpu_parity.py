@@ -107,9 +107,24 @@ def test_tuning_knobs(univ10_dev, univ10_oracle, pw, rpw):
             pats, nv = f()
             check_query(univ10_dev, univ10_oracle, pats, nv)
     finally:
-        univ10_dev.set_tuning(1, 0)
+        univ10_dev.set_tuning(2, 0)
     with pytest.raises(wg.WkError):
         univ10_dev.set_tuning(3, 0)
+
+
+def test_autotune_picks_a_valid_setting(univ10_dev, univ10_oracle, univ10_store):
+    """The native autotuner sweeps every probe width x gather mapping on a
+    batch, keeps the fastest, and the batch stays exact under it."""
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 120, seed=21)
+    qs = [(p, nv) for p, nv, _ in stream]
+    try:
+        pw, rpw, ms = univ10_dev.autotune(qs, use_graph=True)
+        assert pw in (1, 2, 4, 8) and rpw in (0, 1, 4, 16, 64) and ms > 0
+        for (pats, nv), (t, v2c, rc) in zip(qs, univ10_dev.batch(qs, use_graph=True)):
+            want, _, wrc = univ10_oracle.query(pats, nv)
+            assert rc == wrc == 0 and same_rows(t, want), pats
+    finally:
+        univ10_dev.set_tuning(2, 0)
 
 
 @pytest.mark.parametrize("use_graph", [False, True])

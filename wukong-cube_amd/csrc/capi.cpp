@@ -1,6 +1,7 @@
 // extern "C" entry points of include/wk_gpu.h (store, device, tables,
 // per-pattern calls, per-query and batched execution).
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <new>
 
@@ -43,7 +44,7 @@ void free_dev(wk_dev* d) {
 int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   d->st = st;
   d->opts = *o;
-  if (d->opts.probe_width == 0) d->opts.probe_width = 1;
+  if (d->opts.probe_width == 0) d->opts.probe_width = 2;  // fastest measured on the probe at 2,560 universities
   if (d->opts.probe_width != 1 && d->opts.probe_width != 2 && d->opts.probe_width != 4 && d->opts.probe_width != 8)
     return WK_E_INVALID_ARG;
   if (d->opts.rows_per_wave != 1 && d->opts.rows_per_wave != 4 && d->opts.rows_per_wave != 16 &&
@@ -536,6 +537,46 @@ int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph
   int rc = run_batch(dev, plans, 0, nq, 0, use_graph != 0, out->rows, out->offset, out->status, &end);
   dev->batch_used = rc ? 0 : end;
   return rc;
+}
+
+int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, int32_t* probe_width,
+                    int32_t* rows_per_wave, double* batch_ms) {
+  if (!dev || (!qs && nq)) return WK_E_INVALID_ARG;
+  std::vector<Plan> plans(nq);
+  for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, qs[i], plans[i]);
+  std::vector<uint64_t> rows(nq), offs(nq);
+  std::vector<int32_t> status(nq);
+  static const int kWidths[] = {1, 2, 4, 8};
+  static const int kRpw[] = {0, 1, 4, 16, 64};
+  int best_pw = dev->opts.probe_width, best_rpw = dev->opts.rows_per_wave;
+  double best = -1;
+  for (int pw : kWidths) {
+    for (int rpw : kRpw) {
+      int rc = wk_dev_set_tuning(dev, pw, rpw);
+      if (rc) return rc;
+      uint64_t end = 0;
+      // first run builds the graph and warms the caches; the second is timed
+      double ms = 0;
+      for (int rep = 0; rep < 2; rep++) {
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = run_batch(dev, plans, 0, nq, 0, use_graph != 0, rows.data(), offs.data(), status.data(), &end);
+        if (rc) return rc;
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      }
+      if (best < 0 || ms < best) {
+        best = ms;
+        best_pw = pw;
+        best_rpw = rpw;
+      }
+    }
+  }
+  int rc = wk_dev_set_tuning(dev, best_pw, best_rpw);
+  if (rc) return rc;
+  dev->batch_used = 0;
+  if (probe_width) *probe_width = best_pw;
+  if (rows_per_wave) *rows_per_wave = best_rpw;
+  if (batch_ms) *batch_ms = best;
+  return WK_OK;
 }
 
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {

@@ -30,6 +30,16 @@ __device__ __forceinline__ uint32_t edge_at(const StepArgs& a, const DevSeg& s, 
   return a.varena[(blk << a.vshift) | (e & mask)];
 }
 
+// Largest q in [ql, qh) with rs[q] <= g, given rs[ql] <= g.
+__device__ __forceinline__ uint32_t query_in(const uint64_t* rs, uint32_t ql, uint32_t qh, uint64_t g) {
+  while (qh - ql > 1) {
+    const uint32_t mid = (ql + qh) >> 1;
+    if (rs[mid] <= g) ql = mid;
+    else qh = mid;
+  }
+  return ql;
+}
+
 // Largest q in [0, nq) with rs[q] <= g (queries with no rows are skipped).
 __device__ __forceinline__ uint32_t find_query(const uint64_t* rs, uint32_t nq, uint64_t g) {
   uint32_t lo = 0, hi = nq;
@@ -362,7 +372,10 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
 
 // ---- gather ---------------------------------------------------------------
 
-__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
+// Writes output row j of query q: input row g (whose first output is row
+// pg of the step) and, for an expansion, edge eo + (j - pg) of its list.
+__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
+                                         uint32_t eo) {
   const StepDesc& d = a.desc[q];
   const uint64_t r0 = a.row_start[q];
   const uint64_t orow = j - a.pfx[r0];
@@ -372,29 +385,76 @@ __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t
   for (uint32_t c = 0; c < ic; ++c) dst[c] = src[c];
   if (d.op == OP_EXPAND) {
     const DevSeg s = a.segs[d.seg];
-    dst[ic] = edge_at(a, s, uint64_t(a.eoff[g]) + (j - a.pfx[g]));
+    dst[ic] = edge_at(a, s, uint64_t(eo) + (j - pg));
   }
 }
 
-// Output-balanced: one output row per thread; the tile -> first-row map from
-// scan_rows bounds each thread's binary search for its source row.
+__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
+  emit_row(a, q, g, j, a.pfx[g], a.eoff[g]);
+}
+
+// Output-balanced: a block writes one tile of kOutTile consecutive output
+// rows, kOutTile / kBlock per thread in coalesced order.  The tile's source
+// rows [lo, hi) come from the tile -> row map; when they fit, their prefix and
+// edge offsets are staged in LDS and each output row finds its source row by
+// a binary search there (a tile of a heavy row stages one entry).  Tiles of
+// many empty rows (selective filters) search the global prefix instead.
+constexpr int kStage = kOutTile + 1;
+
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   if (a.st->status) return;
+  __shared__ uint64_t s_pfx[kStage];
+  __shared__ uint32_t s_eoff[kStage];
+  __shared__ uint32_t s_q[2];
   const uint64_t R = step_rows(a);
   const uint64_t total = a.st->total_out_rows;
   uint64_t nt = (total + kOutTile - 1) / kOutTile;
   if (nt > a.otile_cap) nt = a.otile_cap;
   for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
-    const uint64_t j = t * kOutTile + threadIdx.x;
-    if (j >= total) continue;
-    uint64_t lo = a.tile_row[t];
-    uint64_t hi = (t + 1 < nt) ? a.tile_row[t + 1] + 1 : R;
-    while (hi - lo > 1) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (a.pfx[mid] <= j) lo = mid;
-      else hi = mid;
+    const uint64_t j0 = t * kOutTile;
+    const uint64_t lo = a.tile_row[t];
+    const uint64_t hi = (t + 1 < nt) ? a.tile_row[t + 1] + 1 : R;
+    const uint64_t nr = hi - lo;
+    const bool staged = nr <= uint64_t(kStage);
+    if (staged) {
+      for (uint32_t i = threadIdx.x; i < uint32_t(nr); i += kBlock) {
+        s_pfx[i] = a.pfx[lo + i];
+        s_eoff[i] = a.eoff[lo + i];
+      }
     }
-    emit_row(a, find_query(a.row_start, a.nq, lo), lo, j);
+    // the queries of rows lo and hi-1 bound every row's query in this tile
+    if (threadIdx.x == 0) {
+      s_q[0] = find_query(a.row_start, a.nq, lo);
+      s_q[1] = find_query(a.row_start, a.nq, hi - 1);
+    }
+    __syncthreads();
+    const uint32_t qlo = s_q[0], qhi = s_q[1] + 1;
+#pragma unroll
+    for (int k = 0; k < kOutTile / kBlock; ++k) {
+      const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+      if (j >= total) continue;
+      // each branch writes its row itself, so no pointer into LDS or global
+      // memory is carried out of it
+      if (staged) {
+        uint32_t l = 0, h = uint32_t(nr);
+        while (h - l > 1) {
+          const uint32_t m = (l + h) >> 1;
+          if (s_pfx[m] <= j) l = m;
+          else h = m;
+        }
+        const uint64_t g = lo + l;
+        emit_row(a, query_in(a.row_start, qlo, qhi, g), g, j, s_pfx[l], s_eoff[l]);
+      } else {
+        uint64_t l = lo, h = hi;
+        while (h - l > 1) {
+          const uint64_t m = (l + h) >> 1;
+          if (a.pfx[m] <= j) l = m;
+          else h = m;
+        }
+        emit_row(a, query_in(a.row_start, qlo, qhi, l), l, j, a.pfx[l], a.eoff[l]);
+      }
+    }
+    __syncthreads();
   }
 }
 

@@ -94,7 +94,7 @@ struct StepArgs {
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kRowsPerThread = 4;
 constexpr int kTile = kBlock * kRowsPerThread;  // rows per probe/scan tile
-constexpr int kOutTile = kBlock;                // output rows per gather tile
+constexpr int kOutTile = 4 * kBlock;            // output rows per gather tile (4 per thread)
 
 // Launch wrappers (all on `stream`; grids sized from host hints, kernels
 // re-read the device sizes and stride).  `rows_hint`/`out_hint` = 0 means

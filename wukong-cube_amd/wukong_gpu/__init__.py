@@ -140,6 +140,8 @@ SIGNATURES = {
     "wk_query_run": (C.c_int, [P, C.POINTER(Query), U64P, U32P, C.POINTER(C.c_int32)]),
     "wk_result_download": (C.c_int, [P, U32P, C.c_uint64]),
     "wk_batch_run": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(BatchResult)]),
+    "wk_dev_autotune": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "wk_batch_download": (C.c_int, [P, U32P, C.c_uint64, U64P]),
     "wk_serve_replay": (C.c_int, [P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
                                   C.POINTER(ServeOpts), C.POINTER(ServeResult)]),
@@ -361,7 +363,7 @@ class Device:
     """HBM gstore cache + history tables + expansion kernels on one GPU."""
 
     def __init__(self, store, device=0, gstore_budget=0, table_budget=0, key_block_shift=0,
-                 value_block_shift=0, enable_pipeline=1, probe_width=1, rows_per_wave=0, profile=0):
+                 value_block_shift=0, enable_pipeline=1, probe_width=0, rows_per_wave=0, profile=0):
         self.store = store
         self._h = P()
         o = DevOpts(device, gstore_budget, table_budget, key_block_shift, value_block_shift,
@@ -480,6 +482,15 @@ class Device:
             t = arena[o:o + r * c].reshape(r, c) if c else np.zeros((0, 0), np.uint32)
             out.append((t.copy(), list(v2c[i * MAX_VARS:i * MAX_VARS + pb.nvars[i]]), int(status[i])))
         return out
+
+    def autotune(self, queries, use_graph=True):
+        """Native sweep of probe width x gather mapping on a batch; keeps the fastest.
+        Returns (probe_width, rows_per_wave, batch_ms)."""
+        pb = queries if isinstance(queries, PreparedBatch) else PreparedBatch(queries)
+        pw, rpw, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().wk_dev_autotune(self._h, pb.qs, pb.n, 1 if use_graph else 0, C.byref(pw), C.byref(rpw),
+                                     C.byref(ms)), "wk_dev_autotune")
+        return pw.value, rpw.value, ms.value
 
     def serve(self, queries, arrivals, max_inflight=4096, footprint_budget=0, per_query=False,
               keep_results=False):
