@@ -14,7 +14,8 @@ void launch_batch_steps(wk_dev* dev, uint32_t S, uint32_t nq, bool timed) {
   for (uint32_t k = 0; k < S; k++) {
     const uint32_t* tin = dev->tbuf[(k + 1) & 1];
     uint32_t* tout = dev->tbuf[k & 1];
-    const StepArgs a = make_args(dev, k, nq, tin, tout, dev->tcap, k + 1 < S);
+    StepArgs a = make_args(dev, k, nq, tin, tout, dev->tcap, k + 1 < S);
+    a.qres = dev->d_qres;
     hipEvent_t ev = nullptr;
     if (timed) timed_begin(dev, &ev);
     launch_probe(a, dev->opts.probe_width, 0, dev->stream);
@@ -110,7 +111,8 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   } else {
     launch_batch_steps(dev, S, nq, dev->opts.profile != 0);
   }
-  if (hip_failed(hipMemcpyAsync(dev->h_desc, dev->d_desc, S * qc * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream)) ||
+  if (hip_failed(hipMemcpyAsync(dev->h_qres, dev->d_qres, size_t(nq) * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                dev->stream)) ||
       hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
       hip_failed(hipStreamSynchronize(dev->stream)))
     return fail(WK_E_HIP);
@@ -134,10 +136,9 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     const uint32_t q = q0 + i;
     const Plan& p = plans[q];
     if (p.status != WK_OK || p.steps.empty()) continue;
-    const StepDesc& d = dev->h_desc[(p.steps.size() - 1) * qc + i];
-    rows[q] = d.out_rows;
-    offsets[q] = d.out_off;
-    end = std::max<uint64_t>(end, d.out_off + d.out_rows * p.cols);
+    rows[q] = dev->h_qres[2 * size_t(i)];
+    offsets[q] = dev->h_qres[2 * size_t(i) + 1];
+    end = std::max<uint64_t>(end, offsets[q] + rows[q] * p.cols);
   }
   *arena_end = end;
   dev->stats.queries_run += nq;

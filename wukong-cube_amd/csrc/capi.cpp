@@ -27,11 +27,12 @@ void free_dev(wk_dev* d) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : d->event_pool) (void)hipEventDestroy(e);
-  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs, d->d_ktab, d->d_vtab, d->tbuf[0], d->tbuf[1], d->arena,
-                      d->cnt,    d->eoff,   d->pfx,    d->tile_sum, d->tile_row, d->d_desc, d->d_rs, d->d_state};
+  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs,   d->d_ktab,    d->d_vtab, d->tbuf[0], d->tbuf[1],
+                      d->arena,  d->cnt,    d->eoff,     d->pfx,       d->tile_sum, d->tile_row, d->d_desc,
+                      d->d_rs,   d->d_state, d->d_qres};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
-  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state};
+  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state, d->h_qres};
   for (void* p : host_ptrs)
     if (p) (void)hipHostFree(p);
   if (d->copy_done) (void)hipEventDestroy(d->copy_done);

@@ -255,6 +255,10 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq) {
   if (dev->h_desc) (void)hipHostFree(dev->h_desc);
   if (dev->h_rs) (void)hipHostFree(dev->h_rs);
   if (dev->h_state) (void)hipHostFree(dev->h_state);
+  if (dev->d_qres) (void)hipFree(dev->d_qres);
+  if (dev->h_qres) (void)hipHostFree(dev->h_qres);
+  dev->d_qres = nullptr;
+  dev->h_qres = nullptr;
   dev->d_desc = nullptr;
   dev->d_rs = nullptr;
   dev->d_state = nullptr;
@@ -269,6 +273,8 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq) {
   WK_HIP(hipHostMalloc(&dev->h_desc, nd * sizeof(StepDesc), hipHostMallocDefault));
   WK_HIP(hipHostMalloc(&dev->h_rs, nr * sizeof(uint64_t), hipHostMallocDefault));
   WK_HIP(hipHostMalloc(&dev->h_state, ns * sizeof(StepState), hipHostMallocDefault));
+  WK_HIP(hipMalloc(&dev->d_qres, size_t(qc) * 2 * sizeof(uint64_t)));
+  WK_HIP(hipHostMalloc(&dev->h_qres, size_t(qc) * 2 * sizeof(uint64_t), hipHostMallocDefault));
   dev->steps_cap = sc;
   dev->nq_cap = qc;
   return WK_OK;
@@ -304,6 +310,7 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
   a.otile_cap = dev->otile_cap;
   a.st = dev->d_state + step;
   a.next_st = has_next ? dev->d_state + step + 1 : nullptr;
+  a.qres = nullptr;
   return a;
 }
 

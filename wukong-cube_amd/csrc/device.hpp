@@ -116,6 +116,8 @@ struct wk_dev {
   wk::StepDesc* h_desc = nullptr;   // pinned staging
   uint64_t* h_rs = nullptr;
   wk::StepState* h_state = nullptr;
+  uint64_t* d_qres = nullptr;       // [nq_cap][2] final rows / arena offset per query
+  uint64_t* h_qres = nullptr;       // pinned staging
   uint32_t nq_cap = 0, steps_cap = 0;
 
   // hipGraph cache for batches: key (steps, nq)
@@ -151,8 +153,6 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq);
 
 StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin, uint32_t* tout,
                    uint64_t tcap, bool has_next);
-void launch_step(wk_dev* dev, const StepArgs& a, uint64_t rows_hint, bool with_gather, uint64_t out_hint,
-                 const char* tag, double alg_bytes);
 
 // Timing helpers (active when opts.profile).
 void timed_begin(wk_dev* dev, hipEvent_t* ev);

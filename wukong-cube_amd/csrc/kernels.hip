@@ -188,6 +188,33 @@ __device__ __forceinline__ ProbeDesc read_probe_desc(const StepDesc* d) {
 
 // ---- probe: per input row, edge count (expand) or keep flag (filters) -----
 
+// One input row: key from the row (or the constant / type index), bucket
+// probe, then the edge count (expand) or the membership flag (filters).  The
+// W lanes of a group call it together with the same row.
+template <int W>
+__device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d, const DevSeg& s,
+                                          const uint32_t* row, uint32_t& c, uint32_t& eo) {
+  // the key's vertex: a column of the row, or the constant start
+  const uint32_t kv = (d.key_mode == KEY_CONST) ? d.kstart : row[d.sc];
+  // KEY_TYPE_INDEX reads [0|t|IN]: only type/predicate ids fit the 17-bit pid field
+  const bool tix = d.key_mode == KEY_TYPE_INDEX;
+  const uint64_t key = tix ? make_key(0, kv, DIR_IN) : make_key(kv, d.pid, d.dir);
+  const bool valid = s.resident != 0 && (!tix || kv < (1u << NBITS_IDX));
+  uint64_t ptr = 0;
+  if (valid && probe<W>(a, s, key, ptr)) {
+    const uint64_t sz = ptr_size(ptr);
+    const uint64_t off = ptr_off(ptr) - s.edge_start;
+    if (d.op == OP_EXPAND) {
+      c = uint32_t(sz);
+      eo = uint32_t(off);
+    } else {
+      uint32_t target = d.kend;
+      if (d.op == OP_FILTER_KNOWN) target = row[d.ec];
+      c = contains(a, s, off, sz, target) ? 1u : 0u;
+    }
+  }
+}
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
   if (a.st->status) return;
@@ -197,39 +224,44 @@ __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
   const int grp = threadIdx.x / W, gl = threadIdx.x % W;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     uint64_t tsum = 0;
-    for (int pass = 0; pass < kTile / G; ++pass) {
-      const uint64_t g = t * kTile + uint64_t(pass) * G + grp;
-      if (g < R) {
-        uint32_t c = 0, eo = 0;
-        const uint32_t q = find_query(a.row_start, a.nq, g);
-        const ProbeDesc d = read_probe_desc(a.desc + q);
-        if (d.op != OP_SKIP && d.seg >= 0) {
-          const DevSeg s = a.segs[d.seg];
-          const uint32_t* row = a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols;
-          // the key's vertex: a column of the row, or the constant start
-          const uint32_t kv = (d.key_mode == KEY_CONST) ? d.kstart : row[d.sc];
-          // KEY_TYPE_INDEX reads [0|t|IN]: only type/predicate ids fit the 17-bit pid field
-          const bool tix = d.key_mode == KEY_TYPE_INDEX;
-          const uint64_t key = tix ? make_key(0, kv, DIR_IN) : make_key(kv, d.pid, d.dir);
-          const bool valid = s.resident != 0 && (!tix || kv < (1u << NBITS_IDX));
-          uint64_t ptr = 0;
-          if (valid && probe<W>(a, s, key, ptr)) {
-            const uint64_t sz = ptr_size(ptr);
-            const uint64_t off = ptr_off(ptr) - s.edge_start;
-            if (d.op == OP_EXPAND) {
-              c = uint32_t(sz);
-              eo = uint32_t(off);
-            } else {
-              uint32_t target = d.kend;
-              if (d.op == OP_FILTER_KNOWN) target = row[d.ec];
-              c = contains(a, s, off, sz, target) ? 1u : 0u;
-            }
+    const uint64_t tlo = t * kTile, thi = (tlo + kTile < R) ? tlo + kTile : R;
+    const uint32_t q0 = find_query(a.row_start, a.nq, tlo), q1 = find_query(a.row_start, a.nq, thi - 1);
+    if (q0 == q1) {
+      // one query owns the tile: descriptor, segment and row base once, in
+      // scalar registers
+      const uint32_t q = __builtin_amdgcn_readfirstlane(q0);
+      const ProbeDesc d = read_probe_desc(a.desc + q);
+      const bool live = d.op != OP_SKIP && d.seg >= 0;
+      const DevSeg s = a.segs[live ? d.seg : 0];
+      const uint32_t* const tbase = a.tin + a.desc[q].in_off;
+      const uint64_t r0 = a.row_start[q];
+      for (int pass = 0; pass < kTile / G; ++pass) {
+        const uint64_t g = tlo + uint64_t(pass) * G + grp;
+        if (g < R) {
+          uint32_t c = 0, eo = 0;
+          if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
+          if (gl == 0) {
+            a.cnt[g] = c;
+            a.eoff[g] = eo;
+            tsum += c;
           }
         }
-        if (gl == 0) {
-          a.cnt[g] = c;
-          a.eoff[g] = eo;
-          tsum += c;
+      }
+    } else {
+      for (int pass = 0; pass < kTile / G; ++pass) {
+        const uint64_t g = tlo + uint64_t(pass) * G + grp;
+        if (g < R) {
+          uint32_t c = 0, eo = 0;
+          const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
+          const ProbeDesc d = read_probe_desc(a.desc + q);
+          if (d.op != OP_SKIP && d.seg >= 0)
+            probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
+                         eo);
+          if (gl == 0) {
+            a.cnt[g] = c;
+            a.eoff[g] = eo;
+            tsum += c;
+          }
         }
       }
     }
@@ -317,50 +349,64 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
     return;
   }
   const uint64_t R = step_rows(a);
-  uint64_t carry_t = a.st->tout_base, carry_a = a.st->arena_base, carry_n = 0;
-  for (uint32_t base = 0; base < a.nq; base += 1024) {
-    const uint32_t q = base + threadIdx.x;
-    uint64_t et = 0, ea = 0, nr = 0, rows = 0;
-    if (q < a.nq) {
-      const StepDesc& d = a.desc[q];
-      uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
-      r0 = r0 < R ? r0 : R;
-      r1 = r1 < R ? r1 : R;
-      rows = a.pfx[r1] - a.pfx[r0];
-      const uint64_t el = rows * d.out_cols;
-      if (d.final_step) {
-        ea = el;
-      } else {
-        et = el;
-        nr = rows;
-      }
+  // each thread owns a contiguous run of queries: three block scans in all
+  const uint32_t per = (a.nq + 1023) / 1024;
+  const uint32_t qlo = threadIdx.x * per < a.nq ? threadIdx.x * per : a.nq;
+  const uint32_t qhi = qlo + per < a.nq ? qlo + per : a.nq;
+  auto rows_of = [&](uint32_t q) {
+    uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
+    r0 = r0 < R ? r0 : R;
+    r1 = r1 < R ? r1 : R;
+    return a.pfx[r1] - a.pfx[r0];
+  };
+  uint64_t et = 0, ea = 0, nr = 0;
+  for (uint32_t q = qlo; q < qhi; ++q) {
+    const uint64_t rows = rows_of(q);
+    const StepDesc& d = a.desc[q];
+    if (d.final_step) {
+      ea += rows * d.out_cols;
+    } else {
+      et += rows * d.out_cols;
+      nr += rows;
     }
-    uint64_t tt = 0, ta = 0, tn = 0;
-    const uint64_t xt = block_excl_scan<1024>(et, &tt);
-    const uint64_t xa = block_excl_scan<1024>(ea, &ta);
-    const uint64_t xn = block_excl_scan<1024>(nr, &tn);
-    if (q < a.nq) {
-      StepDesc& d = a.desc[q];
-      d.out_rows = rows;
-      d.out_off = d.final_step ? carry_a + xa : carry_t + xt;
-      if (a.next) {
-        a.next[q].in_rows = nr;
-        a.next[q].in_off = carry_t + xt;
-        a.next_row_start[q] = carry_n + xn;
-      }
+  }
+  uint64_t tt = 0, ta = 0, tn = 0;
+  const uint64_t tbase = a.st->tout_base, abase = a.st->arena_base;
+  uint64_t xt = block_excl_scan<1024>(et, &tt) + tbase;
+  uint64_t xa = block_excl_scan<1024>(ea, &ta) + abase;
+  uint64_t xn = block_excl_scan<1024>(nr, &tn);
+  for (uint32_t q = qlo; q < qhi; ++q) {
+    const uint64_t rows = rows_of(q);
+    StepDesc& d = a.desc[q];
+    const uint64_t el = rows * d.out_cols;
+    d.out_rows = rows;
+    if (a.next) {
+      a.next[q].in_rows = d.final_step ? 0 : rows;
+      a.next[q].in_off = xt;
+      a.next_row_start[q] = xn;
     }
-    carry_t += tt;
-    carry_a += ta;
-    carry_n += tn;
+    if (d.final_step) {
+      d.out_off = xa;
+      if (a.qres) {
+        a.qres[2 * uint64_t(q)] = rows;
+        a.qres[2 * uint64_t(q) + 1] = xa;
+      }
+      xa += el;
+    } else {
+      d.out_off = xt;
+      xt += el;
+      xn += rows;
+    }
   }
   if (threadIdx.x == 0) {
+    const uint64_t carry_t = tbase + tt, carry_a = abase + ta;
     uint32_t status = 0;
     if (carry_t > a.tcap) status = 1;
     if (carry_a > a.arena_cap) status = 2;
-    a.st->out_elems = carry_t - a.st->tout_base;
+    a.st->out_elems = tt;
     a.st->status = status;
     if (a.next) {
-      a.next_row_start[a.nq] = carry_n;
+      a.next_row_start[a.nq] = tn;
       if (a.next_st) {
         a.next_st->arena_base = carry_a;
         a.next_st->tout_base = 0;
@@ -429,29 +475,60 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
     }
     __syncthreads();
     const uint32_t qlo = s_q[0], qhi = s_q[1] + 1;
+    if (staged && qhi - qlo == 1) {
+      // one query owns the whole tile: its descriptor, row start and segment
+      // are loaded once, into scalar registers
+      const uint32_t q = __builtin_amdgcn_readfirstlane(qlo);
+      const StepDesc& d = a.desc[q];
+      const uint64_t r0 = a.row_start[q];
+      const uint64_t p0 = a.pfx[r0];
+      const uint32_t ic = d.in_cols, oc = d.out_cols;
+      const bool expand = d.op == OP_EXPAND;
+      uint32_t* const dbase = (d.final_step ? a.arena : a.tout) + d.out_off;
+      const uint32_t* const sbase = a.tin + d.in_off;
+      const DevSeg sg = a.segs[expand ? d.seg : 0];
+      uint32_t l = 0;  // source rows only move forward as j grows
 #pragma unroll
-    for (int k = 0; k < kOutTile / kBlock; ++k) {
-      const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
-      if (j >= total) continue;
-      // each branch writes its row itself, so no pointer into LDS or global
-      // memory is carried out of it
-      if (staged) {
-        uint32_t l = 0, h = uint32_t(nr);
+      for (int k = 0; k < kOutTile / kBlock; ++k) {
+        const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+        if (j >= total) continue;
+        uint32_t h = uint32_t(nr);
         while (h - l > 1) {
           const uint32_t m = (l + h) >> 1;
           if (s_pfx[m] <= j) l = m;
           else h = m;
         }
         const uint64_t g = lo + l;
-        emit_row(a, query_in(a.row_start, qlo, qhi, g), g, j, s_pfx[l], s_eoff[l]);
-      } else {
-        uint64_t l = lo, h = hi;
-        while (h - l > 1) {
-          const uint64_t m = (l + h) >> 1;
-          if (a.pfx[m] <= j) l = m;
-          else h = m;
+        uint32_t* dst = dbase + (j - p0) * oc;
+        const uint32_t* src = sbase + (g - r0) * ic;
+        for (uint32_t c = 0; c < ic; ++c) dst[c] = src[c];
+        if (expand) dst[ic] = edge_at(a, sg, uint64_t(s_eoff[l]) + (j - s_pfx[l]));
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kOutTile / kBlock; ++k) {
+        const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+        if (j >= total) continue;
+        // each branch writes its row itself, so no pointer into LDS or global
+        // memory is carried out of it
+        if (staged) {
+          uint32_t l = 0, h = uint32_t(nr);
+          while (h - l > 1) {
+            const uint32_t m = (l + h) >> 1;
+            if (s_pfx[m] <= j) l = m;
+            else h = m;
+          }
+          const uint64_t g = lo + l;
+          emit_row(a, query_in(a.row_start, qlo, qhi, g), g, j, s_pfx[l], s_eoff[l]);
+        } else {
+          uint64_t l = lo, h = hi;
+          while (h - l > 1) {
+            const uint64_t m = (l + h) >> 1;
+            if (a.pfx[m] <= j) l = m;
+            else h = m;
+          }
+          emit_row(a, query_in(a.row_start, qlo, qhi, l), l, j, a.pfx[l], a.eoff[l]);
         }
-        emit_row(a, query_in(a.row_start, qlo, qhi, l), l, j, a.pfx[l], a.eoff[l]);
       }
     }
     __syncthreads();

@@ -89,6 +89,7 @@ struct StepArgs {
   uint64_t otile_cap;
   StepState* st;
   StepState* next_st;      // or null
+  uint64_t* qres;          // [nq][2] rows and arena offset of each final table, or null
 };
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)

@@ -357,6 +357,10 @@ class PreparedBatch:
         self.res = BatchResult(self.rows.ctypes.data_as(U64P), self.cols.ctypes.data_as(U32P),
                                self.status.ctypes.data_as(C.POINTER(C.c_int32)), self.offs.ctypes.data_as(U64P),
                                self.v2c.ctypes.data_as(C.POINTER(C.c_int32)))
+        # the same arrays without the var->column maps (not filled by the library)
+        self.res_rows_only = BatchResult(self.rows.ctypes.data_as(U64P), self.cols.ctypes.data_as(U32P),
+                                         self.status.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         self.offs.ctypes.data_as(U64P), None)
 
 
 class Device:
@@ -469,7 +473,8 @@ class Device:
         (table, v2c, status) or, without fetch, (rows, cols, status) arrays."""
         pb = queries if isinstance(queries, PreparedBatch) else PreparedBatch(queries)
         rows, cols, status, offs, v2c = pb.rows, pb.cols, pb.status, pb.offs, pb.v2c
-        _check(lib().wk_batch_run(self._h, pb.qs, pb.n, 1 if use_graph else 0, C.byref(pb.res)), "wk_batch_run")
+        res = pb.res if fetch else pb.res_rows_only
+        _check(lib().wk_batch_run(self._h, pb.qs, pb.n, 1 if use_graph else 0, C.byref(res)), "wk_batch_run")
         if not fetch:
             return rows.copy(), cols.copy(), status.copy()
         used = C.c_uint64()
