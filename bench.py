@@ -1,21 +1,27 @@
 """Benchmark of the MI355X Wukong GPU query path.
 
-Workload (default): the university-schema graph at 2,560 universities
-(seed 1, ~350M triples, fully resident in HBM).
-  * single-query kernels: Q1 = a 6-pattern cyclic query with a
-    known_to_known closing edge (the lubm_q1 shape with its OSDI16 plan; the
-    lubm_q7 shape is kept off the GPU until a device fault seen with it is
-    explained), Q2 = the 2-pattern type-start query; per-kernel HIP-event
-    timing gives the roofline of the dominant kernel;
+Default workload: the university-schema graph at 2,560 universities (seed 1,
+~339M triples, gstore resident in HBM):
+  * single-query kernels: Q1 = the lubm_q1 shape (6 patterns with its OSDI16
+    plan, a known_to_known join), Q2 = the 2-pattern type-start query.
+    Per-kernel HIP-event timing gives the roofline of the dominant kernel;
   * one "step" = one batched pass over a stream of 4,096 selective queries
-    from 5 templates (constants drawn from the type index), expanded in one
-    set of launches per pattern step over a concatenated table;
+    from 5 templates (constants drawn from the type index).  Each pattern step
+    of all queries is one set of launches over a concatenated table.  The
+    native autotuner first picks the probe width and the gather mapping;
   * the same stream per query (one host sync per pattern) for comparison;
-  * open-loop Poisson serving replay, batched vs per-query, p50/p99;
+  * open-loop Poisson serving replay, continuous batching vs per-query, p50/p99;
   * the C oracle timed on a bounded sample on the host as the CPU baseline.
-Every timed result is checked against the oracle (bit-exact row multisets
-for Q1/Q2, order-independent checksums for the stream).
-Prints one JSON line.
+
+`--workload powerlaw`: the power-law graph (seed 7, 100M vertices, 1.2B
+triples, 256 Zipf(1.1) predicates, Pareto(2) out-degree, 64 types) with the
+star / 5-hop path / snowflake stream, starts drawn in proportion to degree.
+`--gstore-budget-gb` below the store size makes segments swap from pinned
+host memory.  The serving replay runs at the offered rates given by `--rates`.
+
+Every timed result is checked against the oracle: bit-exact row multisets for
+the single queries, row counts and order-independent checksums for the
+stream.  Prints one JSON line.
 """
 import argparse
 import json
@@ -51,63 +57,74 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def main():
+def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("university", "powerlaw"), default="university")
     ap.add_argument("--univ", type=int, default=2560)
+    ap.add_argument("--pl-vertices", type=int, default=100_000_000)
+    ap.add_argument("--pl-triples", type=int, default=1_200_000_000)
     ap.add_argument("--nq", type=int, default=4096)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=None, help="default 1 (university), 7 (power law)")
+    ap.add_argument("--gstore-budget-gb", type=float, default=0, help="0: the whole store")
+    ap.add_argument("--table-budget-gb", type=float, default=48)
+    ap.add_argument("--rates", type=str, default="", help="serving offered rates, q/s, comma separated")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the native autotuner (probe width x gather mapping) before the batch")
-    ap.add_argument("--batch-only", action="store_true", help="only the batched stream (for kernel traces)")
-    ap.add_argument("--probe-width", type=int, default=0, help="lanes per key in the probe (1, 2, 4, 8; 0 = library default)")
+    ap.add_argument("--probe-width", type=int, default=0,
+                    help="lanes per key in the probe (1, 2, 4, 8; 0 = library default)")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="gather mapping: 0 = output-balanced")
     ap.add_argument("--no-graph", action="store_true", help="launch batched steps directly (no hipGraph)")
     ap.add_argument("--single-only", action="store_true",
                     help="only the single-query kernels and their roofline (the launches a kernel trace "
                          "of this command should cover)")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched stream (for kernel traces)")
     args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
-              file=sys.stderr)
-        sys.exit(2)
+    if args.seed is None:
+        args.seed = 1 if args.workload == "university" else 7
+    return args
 
-    import wukong_gpu as wg
-    from wukong_gpu import queries as Q
-    if not os.path.exists(wg.LIB_PATH):
-        raise SystemExit("libwukong_gpu.so is not built: run python __graft_entry__.py")
 
+def make_workload(args, wg, Q):
+    """Triples, single queries, a stream-building function and a description."""
     t0 = time.time()
-    tri = wg.gen_university(args.seed, args.univ)
-    log("generated %d triples for %d universities in %.1fs" % (len(tri), args.univ, time.time() - t0))
-    t0 = time.time()
-    store = wg.Store(tri, pinned=True)
-    info = store.info()
-    log("host gstore built in %.1fs: %s" % (time.time() - t0, info))
-    dev = wg.Device(store, table_budget=48 << 30, profile=1, probe_width=args.probe_width,
-                    rows_per_wave=args.rows_per_wave)
-    t0 = time.time()
-    dev.load_all()
-    st = dev.stats()
-    log("gstore resident in %.1fs (%.2f GB H2D)" % (time.time() - t0, st["h2d_bytes"] / 1e9))
+    if args.workload == "university":
+        tri = wg.gen_university(args.seed, args.univ)
+        singles = {name: getattr(Q, name)() for name in ("lubm_q1", "type_start2")}
+        desc = ("university graph, %d universities (seed %d, %d triples): %d-query template stream batched "
+                "per step; Q1 lubm_q1 shape, Q2 type-start" % (args.univ, args.seed, len(tri), args.nq))
+        cfg = {"universities": args.univ}
 
-    out = {"metric": "queries/sec + p50/p99 latency (batched vs per-query GPU); edges/s; HBM GB/s"}
+        def stream(store):
+            return Q.template_stream(store, Q.univ_templates(), args.nq, seed=args.seed + 100)
+    else:
+        tri = wg.gen_powerlaw(args.seed, args.pl_vertices, args.pl_triples)
+        singles = {}
+        desc = ("power-law graph (seed %d, %d vertices, %d triples, 256 Zipf(1.1) predicates, Pareto(2) "
+                "out-degree): %d-query star/path/snowflake stream, degree-proportional starts" % (
+                    args.seed, args.pl_vertices, len(tri), args.nq))
+        cfg = {"vertices": args.pl_vertices}
 
-    # ---------------------------------------------------------------- single
+        def stream(store):
+            return Q.powerlaw_stream(tri, Q.powerlaw_templates(), args.nq, seed=args.seed + 100)
+    log("generated %d triples in %.1fs" % (len(tri), time.time() - t0))
+    cfg.update({"triples": int(len(tri)), "queries_per_step": args.nq})
+    return tri, singles, stream, desc, cfg
+
+
+def run_single(dev, singles, args):
     single = {}
-    for name in (() if args.batch_only else ("lubm_q1", "type_start2")):
-        pats, nv = getattr(Q, name)()
+    for name, (pats, nv) in singles.items():
         for _ in range(args.warmup):
             dev.query(pats, nv, fetch=False)
         dev.sync()
         dev.reset_kernel_stats()
         lat = []
-        rows = 0
         for _ in range(args.steps):
             t = time.perf_counter()
             _, _, rc = dev.query(pats, nv, fetch=False)
@@ -115,42 +132,76 @@ def main():
             assert rc == 0
         ks = dev.kernel_stats()
         table, v2c, rc = dev.query(pats, nv, fetch=True)
-        rows = table.shape[0]
-        single[name] = {"patterns": len(pats), "rows": int(rows), "cols": int(table.shape[1]),
+        single[name] = {"patterns": len(pats), "rows": int(table.shape[0]), "cols": int(table.shape[1]),
                         "latency_us_mean": float(np.mean(lat)), "latency_us_p50": float(np.median(lat)),
                         "kernels": ks, "_table": table, "_pats": pats, "_nv": nv}
-        log("%s: %d rows, %.0f us/query" % (name, rows, np.mean(lat)))
+        log("%s: %d rows, %.0f us/query" % (name, table.shape[0], np.mean(lat)))
+    return single
 
-    # roofline of the dominant kernel over both single queries
-    roofline = None
-    edges = sum(r["rows"] for r in single.values())
+
+def roofline_of(single, args):
+    """The dominant kernel over the single queries: algorithmic bytes per
+    launch over its average launch time."""
     agg = {}
-    for name, r in single.items():
+    for r in single.values():
         for k, v in r["kernels"].items():
             a = agg.setdefault(k, {"launches": 0, "total_ms": 0.0, "alg_bytes": 0.0})
             for f in a:
                 a[f] += v[f]
-    if agg:
-        dom = max(agg, key=lambda k: agg[k]["total_ms"])
-        d = agg[dom]
-        avg_ms = d["total_ms"] / d["launches"]
-        bytes_per_launch = d["alg_bytes"] / d["launches"]
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        traffic = measured_traffic(dom, "univ%d-single" % args.univ)
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                    "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": d["launches"]}
+    if not agg:
+        return None
+    dom = max(agg, key=lambda k: agg[k]["total_ms"])
+    d = agg[dom]
+    avg_ms = d["total_ms"] / d["launches"]
+    bytes_per_launch = d["alg_bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, "univ%d-single" % args.univ),
+            "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": d["launches"]}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import wukong_gpu as wg
+    from wukong_gpu import queries as Q
+    if not os.path.exists(wg.LIB_PATH):
+        raise SystemExit("libwukong_gpu.so is not built: run python __graft_entry__.py")
+
+    tri, singles, make_stream, desc, cfg = make_workload(args, wg, Q)
+    t0 = time.time()
+    store = wg.Store(tri, pinned=True)
+    info = store.info()
+    log("host gstore built in %.1fs: %s" % (time.time() - t0, info))
+    dev = wg.Device(store, gstore_budget=int(args.gstore_budget_gb * (1 << 30)),
+                    table_budget=int(args.table_budget_gb * (1 << 30)), profile=1,
+                    probe_width=args.probe_width, rows_per_wave=args.rows_per_wave)
+    t0 = time.time()
+    dev.load_all()
+    st = dev.stats()
+    log("gstore loaded in %.1fs (%.2f GB H2D, %d key / %d value blocks free)" % (
+        time.time() - t0, st["h2d_bytes"] / 1e9, st["key_blocks_free"], st["value_blocks_free"]))
+
+    out = {"metric": "queries/sec + p50/p99 latency (batched vs per-query GPU); edges/s; HBM GB/s"}
+
+    # ---------------------------------------------------------------- single
+    single = {} if args.batch_only else run_single(dev, singles, args)
+    roofline = roofline_of(single, args)
+    edges = sum(r["rows"] for r in single.values())
     if args.single_only:
         for r in single.values():
             for k in ("_table", "_pats", "_nv"):
                 r.pop(k, None)
         print(json.dumps({"metric": "single-query kernels", "single_query": single, "roofline": roofline,
-                          "config": {"universities": args.univ, "triples": int(len(tri))}}))
+                          "config": cfg}))
         dev.close()
         return
 
     # ---------------------------------------------------------------- batch
-    stream = Q.template_stream(store, Q.univ_templates(), args.nq, seed=args.seed + 100)
+    stream = make_stream(store)
     qs = [(p, nv) for p, nv, _ in stream]
     pb = wg.PreparedBatch(qs)  # packed once: the timed calls are the C batch entry point
     if not args.no_tune:
@@ -165,22 +216,22 @@ def main():
         rows_b, cols_b, st_b = dev.batch(pb, use_graph=not args.no_graph, fetch=False)
     dev.sync()
     batch_s = (time.perf_counter() - t) / args.steps
-    assert (st_b == 0).all()
+    ok_b = st_b == 0
     qps_batched = args.nq / batch_s
-    log("batched: %.3f ms per %d queries -> %.0f q/s" % (batch_s * 1e3, args.nq, qps_batched))
+    log("batched: %.3f ms per %d queries -> %.0f q/s (%d ok, %d over the table budget, %d result rows)" % (
+        batch_s * 1e3, args.nq, qps_batched, ok_b.sum(), (st_b == wg.WK_E_NOMEM_BUDGET).sum(),
+        int(rows_b[ok_b].sum())))
+    assert ((st_b == 0) | (st_b == wg.WK_E_NOMEM_BUDGET)).all(), np.unique(st_b)
     if args.batch_only:
         print(json.dumps({"metric": "batched queries/s", "value": qps_batched, "ms_per_step": batch_s * 1e3,
-                          "tuning": out.get("tuning"), "config": {"universities": args.univ,
-                                                                   "queries_per_step": args.nq}}))
+                          "tuning": out.get("tuning"), "config": cfg}))
         dev.close()
         return
     # per-query execution of the same stream (bounded sample)
     npq = min(args.nq, 1024)
     t = time.perf_counter()
-    pq_rows = []
     for pats, nv in qs[:npq]:
-        tab, _, rc = dev.query(pats, nv, fetch=False)
-        pq_rows.append(rc)
+        dev.query(pats, nv, fetch=False)
     pq_s = (time.perf_counter() - t) / npq
     qps_perq = 1.0 / pq_s
     log("per-query: %.1f us/query -> %.0f q/s" % (pq_s * 1e6, qps_perq))
@@ -189,16 +240,24 @@ def main():
     serve = {}
     if not args.no_serve:
         rng = np.random.default_rng(args.seed + 7)
-        for mode, rate in (("per_query", 0.7 * qps_perq), ("batched", 10 * 0.7 * qps_perq)):
+        if args.rates:
+            rates = [float(r) for r in args.rates.split(",")]
+            plan = [("per_query", min(rates[0], 0.7 * qps_perq))] + [("batched@%g" % r, r) for r in rates]
+        else:
+            plan = [("per_query", 0.7 * qps_perq), ("batched", 10 * 0.7 * qps_perq)]
+        for mode, rate in plan:
             n = min(len(qs), int(rate * 2.0) + 256)
             arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
             r = dev.serve(qs[:n], arr, max_inflight=4096, per_query=(mode == "per_query"))
             lat = r["latency_us"]
+            ok = r["status"] == 0
             serve[mode] = {"offered_qps": rate, "queries": n, "achieved_qps": n / r["wall_s"],
-                           "p50_us": float(np.percentile(lat, 50)), "p99_us": float(np.percentile(lat, 99)),
-                           "iterations": int(r["batches"]), "_ck": r["checksum"], "_n": n}
-            log("serve %s @ %.0f q/s offered: %.0f q/s, p50 %.0f us, p99 %.0f us" % (
-                mode, rate, n / r["wall_s"], serve[mode]["p50_us"], serve[mode]["p99_us"]))
+                           "p50_us": float(np.percentile(lat[ok], 50)) if ok.any() else None,
+                           "p99_us": float(np.percentile(lat[ok], 99)) if ok.any() else None,
+                           "ok": int(ok.sum()), "iterations": int(r["batches"]), "_ck": r["checksum"],
+                           "_st": r["status"], "_n": n}
+            log("serve %s @ %.0f q/s offered: %.0f q/s, p50 %s us, p99 %s us, %d ok" % (
+                mode, rate, n / r["wall_s"], serve[mode]["p50_us"], serve[mode]["p99_us"], ok.sum()))
 
     # ------------------------------------------------------ CPU baseline + gate
     cpu = None
@@ -215,46 +274,48 @@ def main():
             g = oracle.sorted_rows(r["_table"])
             w = oracle.sorted_rows(want)
             gate[name] = bool(rc == 0 and g.shape == w.shape and np.array_equal(g, w))
-        # stream sample: time the oracle until the budget is spent, checking each result
+        # stream sample: time the oracle until the budget is spent, checking
+        # each result the device returned (queries over the table budget have
+        # no device rows to check)
         t = time.perf_counter()
-        n = 0
+        n = checked = 0
         ok = True
         while n < len(qs) and time.perf_counter() - t < args.cpu_seconds:
             pats, nv = qs[n]
             want, _, rc = orc.query(pats, nv)
             ck = wg.checksum(want) if want.size else 0
-            if rc != 0 or int(rows_b[n]) != want.shape[0]:
+            if rc != 0 or (st_b[n] == 0 and int(rows_b[n]) != want.shape[0]):
                 ok = False
-            if serve and n < serve["batched"]["_n"] and int(serve["batched"]["_ck"][n]) != ck:
-                ok = False
+            for s in serve.values():
+                if n < s["_n"] and s["_st"][n] == 0:
+                    ok = ok and int(s["_ck"][n]) == ck
+                    checked += 1
             n += 1
         cpu_s = time.perf_counter() - t
         gate["stream_sample"] = ok
         cpu = {"value": n / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": "first %d queries of the 4,096-query template stream on the same graph, "
-                         "oracle engine single-threaded (%.1fs)" % (n, cpu_s)}
+               "sample": "first %d queries of the %d-query stream on the same graph, oracle engine "
+                         "single-threaded (%.1fs); %d served results checked" % (n, args.nq, cpu_s, checked)}
         log("cpu oracle: %.0f q/s over %d queries; gate %s" % (n / cpu_s, n, gate))
 
     for r in single.values():
         for k in ("_table", "_pats", "_nv"):
             r.pop(k, None)
     for r in serve.values():
-        r.pop("_ck", None)
-        r.pop("_n", None)
+        for k in ("_ck", "_st", "_n"):
+            r.pop(k, None)
 
     out.update({
         "value": qps_batched, "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": batch_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u32", "data": "synthetic (seeded university-schema generator)",
-        "config": {"workload": "university graph, %d universities (seed %d, %d triples): %d-query template "
-                               "stream batched per step; Q1 lubm_q1 shape, Q2 type-start" % (
-                                   args.univ, args.seed, len(tri), args.nq),
-                   "universities": args.univ, "triples": int(len(tri)), "queries_per_step": args.nq},
+        "dtype": "u32", "data": "synthetic (seeded %s generator)" % args.workload,
+        "config": dict({"workload": desc}, **cfg),
+        "batch_ok": int(ok_b.sum()), "batch_result_rows": int(rows_b[ok_b].sum()),
         "per_query_qps": qps_perq, "batched_vs_per_query": qps_batched / qps_perq,
-        "single_query": single, "edges_per_s": edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values()),
-        "serve": serve,
-        "roofline": roofline,
-        "cpu_baseline": cpu, "parity_gate": gate,
+        "single_query": single,
+        "edges_per_s": (edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values())) if single else None,
+        "serve": serve, "roofline": roofline, "cpu_baseline": cpu, "parity_gate": gate,
+        "gstore": {"h2d_bytes": dev.stats()["h2d_bytes"], "segments_evicted": dev.stats()["segments_evicted"]},
     })
     dev.close()
     print(json.dumps(out))

@@ -21,6 +21,9 @@
 #include "wk_oracle.h"
 
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <string.h>
 
 #define NBITS_IDX 17
@@ -98,14 +101,22 @@ struct orc_store {
   uint64_t n_ivals;
 };
 
-/* LSD radix sort of triples by (a,b,c), 16-bit digits, stable passes. */
+/* LSD radix sort of triples by (a,b,c), 16-bit digits, stable passes.  Each
+ * pass splits the array into one contiguous chunk per thread: per-chunk
+ * digit histograms, then chunk t scatters after chunks 0..t-1 of the same
+ * digit, which keeps every pass stable. */
+static uint32_t trip_field(const trip* x, int field) { return field == 0 ? x->a : field == 1 ? x->b : x->c; }
+
 static int sort_trips(trip* t, uint64_t n) {
   trip* tmp;
   uint64_t* cnt;
-  int pass;
+  int pass, nt = 1;
   if (n < 2) return 0;
+#ifdef _OPENMP
+  nt = n > (1u << 20) ? omp_get_max_threads() : 1;
+#endif
   tmp = (trip*)malloc(n * sizeof(trip));
-  cnt = (uint64_t*)malloc(65536 * sizeof(uint64_t));
+  cnt = (uint64_t*)malloc((size_t)nt * 65536 * sizeof(uint64_t));
   if (!tmp || !cnt) {
     free(tmp);
     free(cnt);
@@ -113,27 +124,41 @@ static int sort_trips(trip* t, uint64_t n) {
   }
   for (pass = 0; pass < 6; pass++) {
     const int field = 2 - pass / 2, shift = (pass & 1) * 16;
-    uint64_t i, run = 0;
     uint32_t orv = 0, andv = 0xffffffffu;
+    uint64_t i, run = 0;
+    int k;
     for (i = 0; i < n; i++) {
-      const uint32_t v = field == 0 ? t[i].a : field == 1 ? t[i].b : t[i].c;
+      const uint32_t v = trip_field(&t[i], field);
       orv |= v;
       andv &= v;
     }
     if ((((orv ^ andv) >> shift) & 0xffffu) == 0) continue; /* digit constant */
-    memset(cnt, 0, 65536 * sizeof(uint64_t));
-    for (i = 0; i < n; i++) {
-      const uint32_t v = field == 0 ? t[i].a : field == 1 ? t[i].b : t[i].c;
-      cnt[(v >> shift) & 0xffffu]++;
-    }
-    for (i = 0; i < 65536; i++) {
-      const uint64_t c = cnt[i];
-      cnt[i] = run;
-      run += c;
-    }
-    for (i = 0; i < n; i++) {
-      const uint32_t v = field == 0 ? t[i].a : field == 1 ? t[i].b : t[i].c;
-      tmp[cnt[(v >> shift) & 0xffffu]++] = t[i];
+    memset(cnt, 0, (size_t)nt * 65536 * sizeof(uint64_t));
+#pragma omp parallel num_threads(nt)
+    {
+      int tid = 0;
+      uint64_t j, lo, hi;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      lo = n * (uint64_t)tid / (uint64_t)nt;
+      hi = n * (uint64_t)(tid + 1) / (uint64_t)nt;
+      for (j = lo; j < hi; j++) cnt[(size_t)tid * 65536 + ((trip_field(&t[j], field) >> shift) & 0xffffu)]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        uint32_t d;
+        for (d = 0; d < 65536; d++)
+          for (k = 0; k < nt; k++) {
+            const uint64_t c = cnt[(size_t)k * 65536 + d];
+            cnt[(size_t)k * 65536 + d] = run;
+            run += c;
+          }
+      }
+      for (j = lo; j < hi; j++) {
+        const uint32_t dg = (trip_field(&t[j], field) >> shift) & 0xffffu;
+        tmp[cnt[(size_t)tid * 65536 + dg]++] = t[j];
+      }
     }
     memcpy(t, tmp, n * sizeof(trip));
   }

@@ -135,27 +135,35 @@ def template_stream(store, templates, n, seed):
 
 def powerlaw_templates(n_preds=256, pred_base=2):
     """star (4 patterns on one subject), 5-hop path, snowflake (8 patterns with
-    two known_to_known joins); predicates are the most frequent (Zipf head)."""
+    two known_to_known joins); predicates are the most frequent (Zipf head).
+    The snowflake's joins check two expanded edges again from their far end
+    (IN direction), so they hold for every row while still probing the IN
+    segments and testing membership on the device."""
     p = [pred_base + i for i in range(n_preds)]
     star = ([(-1, p[0], None, OUT), (-1, p[1], -2, OUT), (-1, p[2], -3, OUT), (-1, p[3], -4, OUT)],
             [(1, "<"), (2, ">"), (3, ">"), (4, ">")], 4)
     path = ([(None, p[0], -1, OUT), (-1, p[1], -2, OUT), (-2, p[0], -3, OUT), (-3, p[2], -4, OUT),
              (-4, p[1], -5, OUT)], [(1, ">"), (2, ">"), (3, ">"), (4, ">"), (5, ">")], 5)
-    snow = ([(None, p[0], -1, OUT), (-1, p[1], -2, OUT), (-1, p[2], -3, OUT), (-2, p[0], -4, OUT),
-             (-3, p[0], -5, OUT), (-2, p[3], -3, OUT), (-4, p[1], -5, OUT), (-1, p[0], -4, OUT)],
-            [(1, ">"), (2, ">"), (3, ">"), (4, ">"), (5, ">"), (6, ">"), (7, ">"), (8, ">")], 5)
+    snow = ([(None, p[0], -1, OUT), (-1, p[1], -2, OUT), (-1, p[2], -3, OUT), (-1, p[1], -2, OUT),
+             (-1, p[2], -3, OUT), (-2, p[0], -4, OUT), (-3, p[3], -5, OUT), (-1, p[4], -6, OUT)],
+            [(1, ">"), (2, ">"), (3, ">"), (4, "<"), (5, "<"), (6, ">"), (7, ">"), (8, ">")], 6)
     return [star, path, snow]
 
 
 def powerlaw_stream(triples, templates, n, seed):
-    """Constant starts sampled in proportion to out-degree (a uniform pick over
-    the triples' subjects)."""
+    """Constant starts sampled in proportion to degree: a uniform pick over the
+    triples of the first pattern's predicate, taking the endpoint that the
+    constant stands for (the subject of (C, p, ?x), the object of (?x, p, C))."""
     rng = np.random.default_rng(seed)
-    subj = triples[:, 0]
+    cand = []
+    for raw, _, _ in templates:
+        s0, p0, o0, _ = raw[0]
+        col = 0 if s0 is None else 2
+        cand.append(triples[triples[:, 1] == p0][:, col])
     out = []
     for _ in range(n):
         k = int(rng.integers(len(templates)))
         raw, plan, nv = templates[k]
-        c = int(subj[rng.integers(len(subj))])
+        c = int(cand[k][rng.integers(len(cand[k]))])
         out.append((set_plan(instantiate(raw, c), plan), nv, k))
     return out
