@@ -228,3 +228,28 @@ def test_serve_results_alone_and_in_mix(univ10_dev, univ10_oracle, univ10_store)
     assert np.array_equal(pq["checksum"], r["checksum"])
     assert np.array_equal(pq["rows"], r["rows"])
     assert r["batches"] < len(qs)
+
+
+def test_serve_restarts_queries_that_do_not_fit_beside_others(univ10_store, univ10_oracle):
+    """Served queries whose step tables do not fit beside the other in-flight
+    queries' tables are restarted alone and come out exact; only a query that
+    cannot fit by itself reports WK_E_NOMEM_BUDGET."""
+    dev = wg.Device(univ10_store, table_budget=4 << 20)   # 149,796 elements per table
+    tcap = dev.stats()["table_capacity_elems"]
+    GS, MEM = Q.U("ub:GraduateStudent"), Q.U("ub:memberOf")
+    depts = np.sort(univ10_store.get_index(Q.U("ub:Department"), wg.IN))[:8]
+    qs = [([(GS, wg.TYPE_ID, -1, wg.IN), (-1, MEM, int(d), wg.OUT)], 1) for d in depts]
+    assert 23151 * len(qs) > tcap and 23151 * 2 < tcap
+    qs.insert(3, Q.type_start2())  # 220,425 x 2 elements alone: cannot fit
+    arr = np.zeros(len(qs))
+    r = dev.serve(qs, arr, max_inflight=64, footprint_budget=1 << 40, keep_results=True)
+    for i, (pats, nv) in enumerate(qs):
+        if i == 3:
+            assert r["status"][i] == wg.WK_E_NOMEM_BUDGET
+            continue
+        assert r["status"][i] == 0, i
+        want, _, _ = univ10_oracle.query(pats, nv)
+        assert want.shape[0] > 0
+        assert same_rows(dev.served_rows(i), want), i
+        assert int(r["checksum"][i]) == wg.checksum(want)
+    dev.close()

@@ -22,6 +22,7 @@ struct QState {
   uint64_t rows = 0;   // rows of the current table
   uint64_t off = 0;    // element offset of the current table in the input buffer
   double admit = 0;
+  uint32_t retries = 0;  // restarts after its step did not fit beside others
 };
 
 // Projected history-table bytes of a query: exact first-pattern cardinality
@@ -227,6 +228,10 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   int cur = 0;
   uint64_t fp_used = 0;
   std::vector<uint32_t> inflight;
+  // queries whose step did not fit the tables beside the other in-flight
+  // queries' steps: restarted from their first pattern, alone
+  std::vector<uint32_t> retry;
+  bool solo = false;  // a restarted query is in flight: admit nothing else
   std::vector<StepDesc> res;
   std::vector<int> failed;
   CkDesc* d_ck = nullptr;
@@ -238,7 +243,19 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   int rc = WK_OK;
   while (ndone < nq && rc == WK_OK) {
     double now = now_s();
-    while (next < nq && arrival_s[next] <= now && inflight.size() < max_inflight) {
+    if (!retry.empty() && inflight.empty()) {
+      const uint32_t q = retry.front();
+      retry.erase(retry.begin());
+      const double admit = qs[q].admit;
+      const uint32_t tries = qs[q].retries;
+      qs[q] = QState{};
+      qs[q].admit = admit;
+      qs[q].retries = tries;
+      fp_used += fp[q];
+      inflight.push_back(q);
+      solo = true;
+    }
+    while (!solo && retry.empty() && next < nq && arrival_s[next] <= now && inflight.size() < max_inflight) {
       const uint32_t q = next;
       if (plans[q].status || plans[q].steps.empty()) {
         qs[q].admit = now;
@@ -304,9 +321,15 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     for (size_t i = 0; i < inflight.size(); i++) {
       const uint32_t q = inflight[i];
       if (std::find(failed.begin(), failed.end(), int(i)) != failed.end()) {
+        fp_used -= fp[q];
+        if (inflight.size() > 1 && qs[q].retries < 3) {
+          // it may fit alone: restart it from its first pattern later
+          qs[q].retries++;
+          retry.push_back(q);
+          continue;
+        }
         finish(q, t, 0, WK_E_NOMEM_BUDGET, 0);
         if (opts->keep_results) dev->served_cols[q] = plans[q].cols;
-        fp_used -= fp[q];
         ndone++;
         continue;
       }
@@ -327,6 +350,7 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
       }
     }
     inflight.swap(still);
+    if (inflight.empty()) solo = false;
     cur ^= 1;
   }
   if (d_ck) (void)hipFree(d_ck);
