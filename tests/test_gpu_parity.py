@@ -253,3 +253,38 @@ def test_serve_restarts_queries_that_do_not_fit_beside_others(univ10_store, univ
         assert same_rows(dev.served_rows(i), want), i
         assert int(r["checksum"][i]) == wg.checksum(want)
     dev.close()
+
+
+def test_known_to_known_on_long_edge_lists():
+    """known_to_known membership tests over edge lists of 2,000-9,000 entries
+    (11-13 levels of the binary search before the 4-entry tail), with probed
+    ids equal to the list's first and last entries, between entries, and
+    outside it."""
+    rng = np.random.default_rng(21)
+    V0 = 1 << 17
+    B, D = 10, 11            # predicate ids
+    hubs = V0 + np.arange(4, dtype=np.uint32)
+    verts = V0 + 100 + np.arange(40000, dtype=np.uint32)
+    rows = []
+    lists = {}
+    for j, h in enumerate(hubs):
+        lst = np.sort(rng.choice(verts, size=2000 + 2300 * j, replace=False))
+        lists[int(h)] = lst
+        rows += [(h, B, v) for v in lst]
+        probe_ids = np.concatenate([rng.choice(verts, size=1500), rng.choice(lst, size=1500),
+                                    lst[:3], lst[-3:], lst[:3] - 1, lst[-3:] + 1, [verts[0], verts[-1]]])
+        rows += [(h, D, int(v)) for v in np.unique(probe_ids)]
+    tri = np.array(rows, dtype=np.uint32)
+    tri = tri[rng.permutation(len(tri))]
+    orc = oracle.Store(tri)
+    dev = wg.Device(wg.Store(tri), table_budget=64 << 20)
+    X, Y = -1, -2
+    pats = [(D, wg.PREDICATE_ID, X, wg.IN), (X, D, Y, wg.OUT), (X, B, Y, wg.OUT)]
+    want = check_query(dev, orc, pats, 2)
+    # the expected rows from the lists themselves
+    exp = 0
+    for h in hubs:
+        ds = tri[(tri[:, 0] == h) & (tri[:, 1] == D), 2]
+        exp += int(np.isin(ds, lists[int(h)]).sum())
+    assert want.shape[0] == exp and exp > 4000
+    dev.close()

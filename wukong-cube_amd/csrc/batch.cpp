@@ -21,11 +21,8 @@ void launch_batch_steps(wk_dev* dev, uint32_t S, uint32_t nq, bool timed) {
     launch_probe(a, dev->opts.probe_width, 0, dev->stream);
     if (timed) timed_end(dev, ev, "batch_probe", 0);
     if (timed) timed_begin(dev, &ev);
-    launch_scan(a, 0, dev->stream);
-    if (timed) timed_end(dev, ev, "batch_scan", 0);
-    if (timed) timed_begin(dev, &ev);
-    launch_gather(a, dev->opts.rows_per_wave, 0, dev->stream);
-    if (timed) timed_end(dev, ev, "batch_gather", 0);
+    launch_expand_tail(a, dev->opts.rows_per_wave, dev->stream);
+    if (timed) timed_end(dev, ev, "batch_scan_gather", 0);
   }
 }
 
@@ -46,33 +43,42 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   int rc = reserve_steps(dev, S, nq);
   if (rc) return rc;
   std::vector<int> segs;
-  for (uint32_t q = q0; q < q1; q++)
-    if (plans[q].status == WK_OK)
-      for (const auto& sd : plans[q].steps)
-        if (sd.seg >= 0 && std::find(segs.begin(), segs.end(), sd.seg) == segs.end()) segs.push_back(sd.seg);
+  {
+    std::vector<uint8_t> seen(dev->seg.size(), 0);
+    for (uint32_t q = q0; q < q1; q++)
+      if (plans[q].status == WK_OK)
+        for (const auto& sd : plans[q].steps)
+          if (sd.seg >= 0 && !seen[size_t(sd.seg)]) {
+            seen[size_t(sd.seg)] = 1;
+            segs.push_back(sd.seg);
+          }
+  }
   rc = ensure_segments(dev, segs);
   if (rc) return rc;
 
+  // descriptors [step][query]: the query's own steps, then skips
   const size_t qc = dev->nq_cap;
-  uint64_t nvalid = 0;
-  for (uint32_t k = 0; k <= S; k++) {
-    for (uint32_t i = 0; i < nq; i++) {
-      const Plan& p = plans[q0 + i];
-      const uint32_t ns = p.status == WK_OK ? uint32_t(p.steps.size()) : 0;
-      StepDesc d{};
+  StepDesc skip{};
+  skip.op = OP_SKIP;
+  skip.seg = -1;
+#pragma omp parallel for schedule(static) if (nq >= 1024)
+  for (uint32_t i = 0; i < nq; i++) {
+    const Plan& p = plans[q0 + i];
+    const uint32_t ns = p.status == WK_OK ? uint32_t(p.steps.size()) : 0;
+    for (uint32_t k = 0; k <= S; k++) {
+      StepDesc* d = dev->h_desc + k * qc + i;
       if (k < ns) {
-        d = p.steps[k];
-        d.final_step = (k + 1 == ns) ? 1 : 0;
+        *d = p.steps[k];
+        d->final_step = (k + 1 == ns) ? 1 : 0;
+        d->in_rows = k == 0 ? 1 : 0;
+        d->in_off = 0;
+        d->out_rows = d->out_off = 0;
       } else {
-        d.op = OP_SKIP;
-        d.seg = -1;
+        *d = skip;
       }
-      d.in_rows = (k == 0 && ns > 0) ? 1 : 0;
-      d.in_off = 0;
-      d.out_rows = d.out_off = 0;
-      dev->h_desc[k * qc + i] = d;
     }
   }
+  uint64_t nvalid = 0;
   for (uint32_t i = 0; i < nq; i++) {
     dev->h_rs[i] = nvalid;
     nvalid += dev->h_desc[i].in_rows;

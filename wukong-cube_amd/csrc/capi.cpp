@@ -29,7 +29,7 @@ void free_dev(wk_dev* d) {
   for (auto e : d->event_pool) (void)hipEventDestroy(e);
   void* dev_ptrs[] = {d->karena, d->varena, d->d_segs,   d->d_ktab,    d->d_vtab, d->tbuf[0], d->tbuf[1],
                       d->arena,  d->cnt,    d->eoff,     d->pfx,       d->tile_sum, d->tile_row, d->d_desc,
-                      d->d_rs,   d->d_state, d->d_qres};
+                      d->d_rs,   d->d_state, d->d_qres, d->d_qagg};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
   void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state, d->h_qres};
@@ -81,12 +81,15 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
     DevSeg& h = d->h_segs[i];
     h = DevSeg{};
     h.num_buckets = m.num_buckets;
+    h.nb_magic = bucket_magic(m.num_buckets);
     h.total_buckets = m.total_buckets();
     h.edge_start = m.edge_start;
     h.num_edges = m.num_edges;
     h.kb_base = uint32_t(kt);
     h.vb_base = uint32_t(vt);
     h.resident = 0;
+    h.kphys = kNoPhys;
+    h.vphys = kNoPhys;
     kt += s.nkb;
     vt += s.nvb;
   }
@@ -525,7 +528,10 @@ int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems) {
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out) {
   if (!dev || (!qs && nq) || !out || !out->rows || !out->cols || !out->status || !out->offset)
     return WK_E_INVALID_ARG;
-  std::vector<Plan> plans(nq);
+  // plans keep their storage from call to call
+  std::vector<Plan>& plans = dev->plan_cache;
+  if (plans.size() < nq) plans.resize(nq);
+#pragma omp parallel for schedule(static) if (nq >= 1024)
   for (uint32_t i = 0; i < nq; i++) {
     compile_query(*dev->st, qs[i], plans[i]);
     out->cols[i] = plans[i].cols;
@@ -543,7 +549,9 @@ int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph
 int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, int32_t* probe_width,
                     int32_t* rows_per_wave, double* batch_ms) {
   if (!dev || (!qs && nq)) return WK_E_INVALID_ARG;
-  std::vector<Plan> plans(nq);
+  std::vector<Plan>& plans = dev->plan_cache;
+  if (plans.size() < nq) plans.resize(nq);
+#pragma omp parallel for schedule(static) if (nq >= 1024)
   for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, qs[i], plans[i]);
   std::vector<uint64_t> rows(nq), offs(nq);
   std::vector<int32_t> status(nq);

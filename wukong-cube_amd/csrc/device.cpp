@@ -132,12 +132,15 @@ namespace {
 
 int evict(wk_dev* dev, int v) {
   SegSlot& s = dev->seg[size_t(v)];
-  for (uint32_t b : s.kblk) dev->free_k.push_back(b);
-  for (uint32_t b : s.vblk) dev->free_v.push_back(b);
+  // pushed last-first, so the next segment loaded takes them in order again
+  for (auto it = s.kblk.rbegin(); it != s.kblk.rend(); ++it) dev->free_k.push_back(*it);
+  for (auto it = s.vblk.rbegin(); it != s.vblk.rend(); ++it) dev->free_v.push_back(*it);
   s.kblk.clear();
   s.vblk.clear();
   s.resident = false;
   dev->h_segs[v].resident = 0;
+  dev->h_segs[v].kphys = kNoPhys;
+  dev->h_segs[v].vphys = kNoPhys;
   WK_HIP(hipMemcpyAsync(dev->d_segs + v, dev->h_segs + v, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
   dev->stats.segments_evicted++;
@@ -202,6 +205,13 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
   if (s.nvb)
     WK_HIP(hipMemcpyAsync(dev->d_vtab + hs.vb_base, dev->h_vtab + hs.vb_base, s.nvb * sizeof(uint32_t),
                           hipMemcpyHostToDevice, dev->copy_stream));
+  auto first_if_consecutive = [](const std::vector<uint32_t>& blk) {
+    for (size_t j = 1; j < blk.size(); j++)
+      if (blk[j] != blk[0] + j) return kNoPhys;
+    return blk.empty() ? 0u : blk[0];
+  };
+  hs.kphys = first_if_consecutive(s.kblk);
+  hs.vphys = first_if_consecutive(s.vblk);
   hs.resident = 1;
   WK_HIP(hipMemcpyAsync(dev->d_segs + si, dev->h_segs + si, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
@@ -257,6 +267,8 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq) {
   if (dev->h_state) (void)hipHostFree(dev->h_state);
   if (dev->d_qres) (void)hipFree(dev->d_qres);
   if (dev->h_qres) (void)hipHostFree(dev->h_qres);
+  if (dev->d_qagg) (void)hipFree(dev->d_qagg);
+  dev->d_qagg = nullptr;
   dev->d_qres = nullptr;
   dev->h_qres = nullptr;
   dev->d_desc = nullptr;
@@ -275,6 +287,7 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq) {
   WK_HIP(hipHostMalloc(&dev->h_state, ns * sizeof(StepState), hipHostMallocDefault));
   WK_HIP(hipMalloc(&dev->d_qres, size_t(qc) * 2 * sizeof(uint64_t)));
   WK_HIP(hipHostMalloc(&dev->h_qres, size_t(qc) * 2 * sizeof(uint64_t), hipHostMallocDefault));
+  WK_HIP(hipMalloc(&dev->d_qagg, (size_t(qc) + kBlock - 1) / kBlock * 3 * sizeof(uint64_t)));
   dev->steps_cap = sc;
   dev->nq_cap = qc;
   return WK_OK;
@@ -311,6 +324,7 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
   a.st = dev->d_state + step;
   a.next_st = has_next ? dev->d_state + step + 1 : nullptr;
   a.qres = nullptr;
+  a.qagg = dev->d_qagg;
   return a;
 }
 

@@ -118,7 +118,11 @@ struct wk_dev {
   wk::StepState* h_state = nullptr;
   uint64_t* d_qres = nullptr;       // [nq_cap][2] final rows / arena offset per query
   uint64_t* h_qres = nullptr;       // pinned staging
+  uint64_t* d_qagg = nullptr;       // [ceil(nq_cap / 256)][3] query-scan block sums
   uint32_t nq_cap = 0, steps_cap = 0;
+
+  // compiled plans of the last batch (storage reused by the next)
+  std::vector<wk::Plan> plan_cache;
 
   // hipGraph cache for batches: key (steps, nq)
   std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs;

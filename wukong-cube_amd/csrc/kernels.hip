@@ -18,16 +18,31 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kPersistentGrid = 2048;
 
+// A segment whose blocks sit consecutively in the arena (every segment of a
+// store that fits the budget) is addressed directly; otherwise through its
+// block table, one more dependent load.
+// Both choose an arena index, not a pointer: the two ways meet on an integer,
+// so no pointer phi reaches the load.
 __device__ __forceinline__ const Slot* bucket_addr(const StepArgs& a, const DevSeg& s, uint64_t b) {
-  const uint64_t mask = (uint64_t(1) << a.kshift) - 1;
-  const uint64_t blk = a.ktab[s.kb_base + (b >> a.kshift)];
-  return a.karena + ((blk << a.kshift) | (b & mask)) * ASSOCIATIVITY;
+  uint64_t i;
+  if (s.kphys != kNoPhys) {
+    i = (uint64_t(s.kphys) << a.kshift) + b;
+  } else {
+    const uint64_t blk = a.ktab[s.kb_base + (b >> a.kshift)];
+    i = (blk << a.kshift) | (b & ((uint64_t(1) << a.kshift) - 1));
+  }
+  return a.karena + i * ASSOCIATIVITY;
 }
 
 __device__ __forceinline__ uint32_t edge_at(const StepArgs& a, const DevSeg& s, uint64_t e) {
-  const uint64_t mask = (uint64_t(1) << a.vshift) - 1;
-  const uint64_t blk = a.vtab[s.vb_base + (e >> a.vshift)];
-  return a.varena[(blk << a.vshift) | (e & mask)];
+  uint64_t i;
+  if (s.vphys != kNoPhys) {
+    i = (uint64_t(s.vphys) << a.vshift) + e;
+  } else {
+    const uint64_t blk = a.vtab[s.vb_base + (e >> a.vshift)];
+    i = (blk << a.vshift) | (e & ((uint64_t(1) << a.vshift) - 1));
+  }
+  return a.varena[i];
 }
 
 // Largest q in [ql, qh) with rs[q] <= g, given rs[ql] <= g.
@@ -49,6 +64,42 @@ __device__ __forceinline__ uint32_t find_query(const uint64_t* rs, uint32_t nq, 
     else hi = mid;
   }
   return lo;
+}
+
+// find_query for two values g0 <= g1 at once, by all NT threads of a block:
+// each round reads NT evenly spaced entries of the current interval together
+// (one memory latency) and keeps the stretch between the two samples around
+// g, so up to NT queries take one round instead of log2(nq) dependent loads.
+// Every thread of the block must call it (it synchronizes the block).
+template <int NT>
+__device__ __forceinline__ void block_find_queries(const uint64_t* rs, uint32_t nq, uint64_t g0, uint64_t g1,
+                                                   uint32_t& q0, uint32_t& q1) {
+  __shared__ uint32_t s_at[2][2];
+  uint32_t lo[2] = {0, 0}, len[2] = {nq, nq};
+  const uint64_t g[2] = {g0, g1};
+  for (int round = 0; len[0] > 1 || len[1] > 1; ++round) {
+    uint32_t st[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      st[k] = (len[k] + NT - 1) / NT;
+      const uint64_t o = uint64_t(threadIdx.x) * st[k];
+      // the one sample <= g whose successor sample is > g or past the end
+      if (len[k] > 1 && o < len[k] && rs[lo[k] + o] <= g[k] &&
+          (o + st[k] >= len[k] || rs[lo[k] + o + st[k]] > g[k]))
+        s_at[round & 1][k] = threadIdx.x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (len[k] > 1) {
+        const uint32_t o = s_at[round & 1][k] * st[k];
+        lo[k] += o;
+        len[k] = (st[k] < len[k] - o) ? st[k] : len[k] - o;
+      }
+    }
+  }
+  q0 = lo[0];
+  q1 = lo[1];
 }
 
 template <typename T>
@@ -98,7 +149,7 @@ __device__ __forceinline__ bool probe(const StepArgs& a, const DevSeg& s, uint64
   const int lane = threadIdx.x & (kWave - 1);
   const int gl = lane & (W - 1);
   const int gbase = lane & ~(W - 1);
-  uint64_t b = hash_u64(key) % s.num_buckets;
+  uint64_t b = bucket_of(hash_u64(key), s.num_buckets, s.nb_magic);
   for (uint64_t hop = 0; hop < s.total_buckets; ++hop) {
     const uint4* bk = reinterpret_cast<const uint4*>(bucket_addr(a, s, b));
     bool hit = false;
@@ -135,20 +186,45 @@ __device__ __forceinline__ bool probe(const StepArgs& a, const DevSeg& s, uint64
   return false;
 }
 
-// Membership of x in the sorted edge list [off, off+n) of segment s.
-__device__ __forceinline__ bool contains(const StepArgs& a, const DevSeg& s, uint64_t off, uint64_t n,
-                                         uint32_t x) {
+// Membership of x in the sorted, duplicate-free list of n edges read by
+// at(i): binary search down to <= 4 entries, which are then read together
+// (independent loads, one memory latency).  A wider multi-pivot search costs
+// the probe kernels enough registers to lower their occupancy, and the lists
+// probed here are mostly short.
+template <typename At>
+__device__ __forceinline__ bool search_sorted(At at, uint64_t n, uint32_t x) {
   uint64_t lo = 0, hi = n;
-  while (hi - lo > 8) {
+  while (hi - lo > 4) {
     const uint64_t mid = (lo + hi) >> 1;
-    const uint32_t v = edge_at(a, s, off + mid);
+    const uint32_t v = at(mid);
     if (v < x) lo = mid + 1;
     else if (v > x) hi = mid;
     else return true;
   }
-  for (uint64_t i = lo; i < hi; ++i)
-    if (edge_at(a, s, off + i) == x) return true;
-  return false;
+  bool found = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (lo + k < hi) found = found || at(lo + k) == x;
+  return found;
+}
+
+// Membership of x in the edge list [off, off+n) of segment s: directly in
+// the arena when the segment's value blocks are consecutive, else through
+// the block table.  Each way keeps its addressing to itself, so only the
+// boolean answer leaves the branch.
+__device__ __forceinline__ bool contains(const StepArgs& a, const DevSeg& s, uint64_t off, uint64_t n,
+                                         uint32_t x) {
+  if (s.vphys != kNoPhys) {
+    const uint32_t* e = a.varena + (uint64_t(s.vphys) << a.vshift) + off;
+    return search_sorted([e](uint64_t i) { return e[i]; }, n, x);
+  }
+  const uint64_t mask = (uint64_t(1) << a.vshift) - 1;
+  return search_sorted(
+      [&a, &s, off, mask](uint64_t i) {
+        const uint64_t g = off + i;
+        return a.varena[(uint64_t(a.vtab[s.vb_base + (g >> a.vshift)]) << a.vshift) | (g & mask)];
+      },
+      n, x);
 }
 
 __device__ __forceinline__ uint64_t step_rows(const StepArgs& a) {
@@ -215,17 +291,29 @@ __device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d,
   }
 }
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
+// One block of NT threads per tile of kTile rows, in kTile / (NT / W)
+// passes.  A tile inside one query reads the descriptor and segment once,
+// into scalar registers; a tile spanning several queries stages their row
+// starts in LDS and each row finds its query there.  NT = 1024 (one row per
+// lane at W = 1) serves batches, whose tiles mix queries; NT = 256 serves one
+// large query, where a smaller workgroup lets a finished one be replaced
+// sooner.  (Capping the registers for one more wave per SIMD made the large
+// query's probe 30% slower: fewer loads stay in flight per wave.)
+template <int W, int NT>
+__global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
+  static_assert(kTile % NT == 0 && NT % W == 0, "probe tile shape");
   if (a.st->status) return;
+  __shared__ uint64_t s_rs[NT];
   const uint64_t R = step_rows(a);
   const uint64_t ntiles = (R + kTile - 1) / kTile;
-  constexpr int G = kBlock / W;
+  constexpr int G = NT / W;
+  constexpr int kPasses = kTile / G;
   const int grp = threadIdx.x / W, gl = threadIdx.x % W;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     uint64_t tsum = 0;
     const uint64_t tlo = t * kTile, thi = (tlo + kTile < R) ? tlo + kTile : R;
-    const uint32_t q0 = find_query(a.row_start, a.nq, tlo), q1 = find_query(a.row_start, a.nq, thi - 1);
+    uint32_t q0, q1;
+    block_find_queries<NT>(a.row_start, a.nq, tlo, thi - 1, q0, q1);
     if (q0 == q1) {
       // one query owns the tile: descriptor, segment and row base once, in
       // scalar registers
@@ -235,9 +323,9 @@ __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
       const DevSeg s = a.segs[live ? d.seg : 0];
       const uint32_t* const tbase = a.tin + a.desc[q].in_off;
       const uint64_t r0 = a.row_start[q];
-      for (int pass = 0; pass < kTile / G; ++pass) {
+      for (int pass = 0; pass < kPasses; ++pass) {
         const uint64_t g = tlo + uint64_t(pass) * G + grp;
-        if (g < R) {
+        if (g < thi) {
           uint32_t c = 0, eo = 0;
           if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
           if (gl == 0) {
@@ -248,11 +336,26 @@ __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
         }
       }
     } else {
-      for (int pass = 0; pass < kTile / G; ++pass) {
+      const uint32_t nqt = q1 - q0 + 1;
+      const bool staged = nqt <= uint32_t(NT);
+      if (staged && threadIdx.x < nqt) s_rs[threadIdx.x] = a.row_start[q0 + threadIdx.x];
+      __syncthreads();
+      for (int pass = 0; pass < kPasses; ++pass) {
         const uint64_t g = tlo + uint64_t(pass) * G + grp;
-        if (g < R) {
+        if (g < thi) {
           uint32_t c = 0, eo = 0;
-          const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
+          uint32_t q;
+          if (staged) {
+            uint32_t l = 0, h = nqt;
+            while (h - l > 1) {
+              const uint32_t m = (l + h) >> 1;
+              if (s_rs[m] <= g) l = m;
+              else h = m;
+            }
+            q = q0 + l;
+          } else {
+            q = query_in(a.row_start, q0, q1 + 1, g);
+          }
           const ProbeDesc d = read_probe_desc(a.desc + q);
           if (d.op != OP_SKIP && d.seg >= 0)
             probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
@@ -265,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_probe(StepArgs a) {
         }
       }
     }
-    tsum = block_sum<kBlock>(tsum);
+    tsum = block_sum<NT>(tsum);
     if (threadIdx.x == 0 && t < a.tile_cap) a.tile_sum[t] = tsum;
   }
 }
@@ -359,15 +462,56 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
     r1 = r1 < R ? r1 : R;
     return a.pfx[r1] - a.pfx[r0];
   };
+  // up to kPer queries per thread (nq <= 8192): their row counts are loaded
+  // together and kept in registers for the second pass
+  constexpr uint32_t kPer = 8;
+  const bool cached = per <= kPer;
+  uint64_t crows[kPer];
+  uint32_t ccols[kPer];  // out_cols, or 0x80000000 | out_cols for a final step
   uint64_t et = 0, ea = 0, nr = 0;
-  for (uint32_t q = qlo; q < qhi; ++q) {
-    const uint64_t rows = rows_of(q);
-    const StepDesc& d = a.desc[q];
-    if (d.final_step) {
-      ea += rows * d.out_cols;
-    } else {
-      et += rows * d.out_cols;
-      nr += rows;
+  if (cached) {
+    // every load is unconditional at a clamped, in-bounds index, so they all
+    // issue back to back; entries past the thread's run are masked after
+    uint64_t rs[kPer + 1];
+#pragma unroll
+    for (uint32_t i = 0; i <= kPer; ++i) {
+      const uint32_t qi = qlo + i < qhi ? qlo + i : qhi;  // row_start has nq + 1 entries
+      const uint64_t r = a.row_start[qi];
+      rs[i] = r < R ? r : R;
+    }
+    uint64_t pf[kPer + 1];
+#pragma unroll
+    for (uint32_t i = 0; i <= kPer; ++i) pf[i] = a.pfx[rs[i]];
+    uint32_t dc[kPer], df[kPer];
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t qi = qlo + i < a.nq ? qlo + i : a.nq - 1;
+      dc[i] = a.desc[qi].out_cols;
+      df[i] = a.desc[qi].final_step;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const bool in = qlo + i < qhi;
+      const uint64_t rows = in ? pf[i + 1] - pf[i] : 0;
+      crows[i] = rows;
+      ccols[i] = in ? (dc[i] | (df[i] ? 0x80000000u : 0u)) : 0u;
+      if (df[i]) {
+        ea += rows * dc[i];
+      } else {
+        et += rows * dc[i];
+        nr += rows;
+      }
+    }
+  } else {
+    for (uint32_t q = qlo; q < qhi; ++q) {
+      const uint64_t rows = rows_of(q);
+      const StepDesc& d = a.desc[q];
+      if (d.final_step) {
+        ea += rows * d.out_cols;
+      } else {
+        et += rows * d.out_cols;
+        nr += rows;
+      }
     }
   }
   uint64_t tt = 0, ta = 0, tn = 0;
@@ -376,16 +520,31 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
   uint64_t xa = block_excl_scan<1024>(ea, &ta) + abase;
   uint64_t xn = block_excl_scan<1024>(nr, &tn);
   for (uint32_t q = qlo; q < qhi; ++q) {
-    const uint64_t rows = rows_of(q);
+    uint64_t rows = 0;
+    uint32_t oc = 0;
+    bool fin = false;
+    if (cached) {
+#pragma unroll
+      for (uint32_t i = 0; i < kPer; ++i)
+        if (q - qlo == i) {
+          rows = crows[i];
+          oc = ccols[i] & 0x7FFFFFFFu;
+          fin = (ccols[i] >> 31) != 0;
+        }
+    } else {
+      rows = rows_of(q);
+      oc = a.desc[q].out_cols;
+      fin = a.desc[q].final_step != 0;
+    }
     StepDesc& d = a.desc[q];
-    const uint64_t el = rows * d.out_cols;
+    const uint64_t el = rows * oc;
     d.out_rows = rows;
     if (a.next) {
-      a.next[q].in_rows = d.final_step ? 0 : rows;
+      a.next[q].in_rows = fin ? 0 : rows;
       a.next[q].in_off = xt;
       a.next_row_start[q] = xn;
     }
-    if (d.final_step) {
+    if (fin) {
       d.out_off = xa;
       if (a.qres) {
         a.qres[2 * uint64_t(q)] = rows;
@@ -416,15 +575,155 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
   }
 }
 
+// Multi-block form of k_scan_queries for large batches.  One block scanning
+// thousands of queries is bound by its own scattered stores (each query's
+// descriptor and next-step descriptor are separate cache lines), so the
+// queries are spread over ceil(nq / kBlock) blocks in two launches.
+//
+// First launch: blocks [0, nqb) sum the non-final output elements, final
+// output elements and next-step rows of their kBlock queries into qagg; the
+// blocks after them mark output tiles (the work of k_tile_rows), which needs
+// only the prefix computed before this launch.
+__global__ __launch_bounds__(kBlock) void k_query_sums(StepArgs a, uint32_t nqb) {
+  if (a.st->status) return;
+  const uint64_t R = step_rows(a);
+  if (blockIdx.x < nqb) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    uint64_t et = 0, ea = 0, nr = 0;
+    if (q < a.nq) {
+      uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
+      r0 = r0 < R ? r0 : R;
+      r1 = r1 < R ? r1 : R;
+      const uint64_t rows = a.pfx[r1] - a.pfx[r0];
+      const uint32_t oc = a.desc[q].out_cols;
+      if (a.desc[q].final_step) {
+        ea = rows * oc;
+      } else {
+        et = rows * oc;
+        nr = rows;
+      }
+    }
+    et = block_sum<kBlock>(et);
+    ea = block_sum<kBlock>(ea);
+    nr = block_sum<kBlock>(nr);
+    if (threadIdx.x == 0) {
+      a.qagg[3 * uint64_t(blockIdx.x)] = et;
+      a.qagg[3 * uint64_t(blockIdx.x) + 1] = ea;
+      a.qagg[3 * uint64_t(blockIdx.x) + 2] = nr;
+    }
+    return;
+  }
+  uint64_t nt = (a.st->total_out_rows + kOutTile - 1) / kOutTile;
+  if (nt > a.otile_cap) nt = a.otile_cap;
+  const uint64_t nb = gridDim.x - nqb;
+  for (uint64_t t = uint64_t(blockIdx.x - nqb) * kBlock + threadIdx.x; t < nt; t += nb * kBlock) {
+    const uint64_t j = t * kOutTile;
+    uint64_t lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (a.pfx[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    a.tile_row[t] = lo;
+  }
+}
+
+// Second launch: each block adds the sums of the blocks before it (and the
+// totals of all of them) to its own block scans, then writes its queries'
+// offsets exactly as k_scan_queries does; the last block writes the step's
+// totals and status.
+__global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t nqb) {
+  if (a.st->status) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.next_st) a.next_st->status = a.st->status;
+    return;
+  }
+  __shared__ uint64_t s_red[6][kBlock / kWave];
+  const uint64_t R = step_rows(a);
+  uint64_t v[6] = {0, 0, 0, 0, 0, 0};  // prefix t/a/n, then totals t/a/n
+  for (uint32_t i = threadIdx.x; i < nqb; i += kBlock) {
+    const uint64_t x = a.qagg[3 * uint64_t(i)], y = a.qagg[3 * uint64_t(i) + 1], z = a.qagg[3 * uint64_t(i) + 2];
+    if (i < blockIdx.x) {
+      v[0] += x;
+      v[1] += y;
+      v[2] += z;
+    }
+    v[3] += x;
+    v[4] += y;
+    v[5] += z;
+  }
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const uint64_t x = wave_incl_scan<unsigned long long>(v[k]);
+    if (lane == kWave - 1) s_red[k][w] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / kWave; ++i) t += s_red[k][i];
+    v[k] = t;
+  }
+  const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+  uint64_t rows = 0;
+  uint32_t oc = 0;
+  bool fin = false;
+  if (q < a.nq) {
+    uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
+    r0 = r0 < R ? r0 : R;
+    r1 = r1 < R ? r1 : R;
+    rows = a.pfx[r1] - a.pfx[r0];
+    oc = a.desc[q].out_cols;
+    fin = a.desc[q].final_step != 0;
+  }
+  const uint64_t el = rows * oc;
+  const uint64_t tbase = a.st->tout_base, abase = a.st->arena_base;
+  const uint64_t xt = block_excl_scan<kBlock>(fin ? 0 : el, nullptr) + v[0] + tbase;
+  const uint64_t xa = block_excl_scan<kBlock>(fin ? el : 0, nullptr) + v[1] + abase;
+  const uint64_t xn = block_excl_scan<kBlock>(fin ? 0 : rows, nullptr) + v[2];
+  if (q < a.nq) {
+    StepDesc& d = a.desc[q];
+    d.out_rows = rows;
+    d.out_off = fin ? xa : xt;
+    if (a.next) {
+      a.next[q].in_rows = fin ? 0 : rows;
+      a.next[q].in_off = xt;
+      a.next_row_start[q] = xn;
+    }
+    if (fin && a.qres) {
+      a.qres[2 * uint64_t(q)] = rows;
+      a.qres[2 * uint64_t(q) + 1] = xa;
+    }
+  }
+  if (blockIdx.x == nqb - 1 && threadIdx.x == 0) {
+    const uint64_t tt = v[3], ta = v[4], tn = v[5];
+    const uint64_t carry_t = tbase + tt, carry_a = abase + ta;
+    uint32_t status = 0;
+    if (carry_t > a.tcap) status = 1;
+    if (carry_a > a.arena_cap) status = 2;
+    a.st->out_elems = tt;
+    a.st->status = status;
+    if (a.next) {
+      a.next_row_start[a.nq] = tn;
+      if (a.next_st) {
+        a.next_st->arena_base = carry_a;
+        a.next_st->tout_base = 0;
+        a.next_st->status = status;
+      }
+    }
+  }
+}
+
 // ---- gather ---------------------------------------------------------------
 
 // Writes output row j of query q: input row g (whose first output is row
 // pg of the step) and, for an expansion, edge eo + (j - pg) of its list.
-__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
-                                         uint32_t eo) {
+// r0 is the query's first input row and p0 = pfx[r0] its first output row.
+__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t r0, uint64_t p0, uint64_t g,
+                                         uint64_t j, uint64_t pg, uint32_t eo) {
   const StepDesc& d = a.desc[q];
-  const uint64_t r0 = a.row_start[q];
-  const uint64_t orow = j - a.pfx[r0];
+  const uint64_t orow = j - p0;
   uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
   const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
   const uint32_t ic = d.in_cols;
@@ -433,6 +732,12 @@ __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t
     const DevSeg s = a.segs[d.seg];
     dst[ic] = edge_at(a, s, uint64_t(eo) + (j - pg));
   }
+}
+
+__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
+                                         uint32_t eo) {
+  const uint64_t r0 = a.row_start[q];
+  emit_row(a, q, r0, a.pfx[r0], g, j, pg, eo);
 }
 
 __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
@@ -446,12 +751,14 @@ __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t
 // a binary search there (a tile of a heavy row stages one entry).  Tiles of
 // many empty rows (selective filters) search the global prefix instead.
 constexpr int kStage = kOutTile + 1;
+constexpr int kQStage = kBlock;  // queries of a mixed tile staged in LDS
 
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   if (a.st->status) return;
   __shared__ uint64_t s_pfx[kStage];
   __shared__ uint32_t s_eoff[kStage];
-  __shared__ uint32_t s_q[2];
+  __shared__ uint64_t s_qr0[kQStage];  // first input row of each query of the tile
+  __shared__ uint64_t s_qp0[kQStage];  // and its first output row
   const uint64_t R = step_rows(a);
   const uint64_t total = a.st->total_out_rows;
   uint64_t nt = (total + kOutTile - 1) / kOutTile;
@@ -463,18 +770,31 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
     const uint64_t nr = hi - lo;
     const bool staged = nr <= uint64_t(kStage);
     if (staged) {
-      for (uint32_t i = threadIdx.x; i < uint32_t(nr); i += kBlock) {
-        s_pfx[i] = a.pfx[lo + i];
-        s_eoff[i] = a.eoff[lo + i];
+      // unconditional loads at clamped indices (rows lo..hi-1 exist), so all
+      // of them are in flight together
+#pragma unroll
+      for (int k = 0; k < (kStage + kBlock - 1) / kBlock; ++k) {
+        const uint32_t i = threadIdx.x + k * kBlock;
+        const uint64_t gi = lo + (i < uint32_t(nr) ? i : uint32_t(nr) - 1);
+        const uint64_t pv = a.pfx[gi];
+        const uint32_t ev = a.eoff[gi];
+        if (i < uint32_t(nr)) {
+          s_pfx[i] = pv;
+          s_eoff[i] = ev;
+        }
       }
     }
     // the queries of rows lo and hi-1 bound every row's query in this tile
-    if (threadIdx.x == 0) {
-      s_q[0] = find_query(a.row_start, a.nq, lo);
-      s_q[1] = find_query(a.row_start, a.nq, hi - 1);
+    uint32_t qlo, qlast;
+    block_find_queries<kBlock>(a.row_start, a.nq, lo, hi - 1, qlo, qlast);
+    const uint32_t qhi = qlast + 1;
+    const bool qstaged = qhi - qlo <= uint32_t(kQStage);
+    if (qhi - qlo > 1 && qstaged && threadIdx.x < qhi - qlo) {
+      const uint64_t r0 = a.row_start[qlo + threadIdx.x];
+      s_qr0[threadIdx.x] = r0;
+      s_qp0[threadIdx.x] = a.pfx[r0 < R ? r0 : R];
     }
     __syncthreads();
-    const uint32_t qlo = s_q[0], qhi = s_q[1] + 1;
     if (staged && qhi - qlo == 1) {
       // one query owns the whole tile: its descriptor, row start and segment
       // are loaded once, into scalar registers
@@ -511,7 +831,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
         if (j >= total) continue;
         // each branch writes its row itself, so no pointer into LDS or global
         // memory is carried out of it
-        if (staged) {
+        if (staged && qstaged) {
+          uint32_t l = 0, h = uint32_t(nr);
+          while (h - l > 1) {
+            const uint32_t m = (l + h) >> 1;
+            if (s_pfx[m] <= j) l = m;
+            else h = m;
+          }
+          const uint64_t g = lo + l;
+          uint32_t ql = 0, qh = qhi - qlo;
+          while (qh - ql > 1) {
+            const uint32_t m = (ql + qh) >> 1;
+            if (s_qr0[m] <= g) ql = m;
+            else qh = m;
+          }
+          emit_row(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, s_pfx[l], s_eoff[l]);
+        } else if (staged) {
           uint32_t l = 0, h = uint32_t(nr);
           while (h - l > 1) {
             const uint32_t m = (l + h) >> 1;
@@ -612,15 +947,23 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
 
 }  // namespace
 
+template <int W>
+void probe_launch(const StepArgs& a, unsigned grid, bool one_query, hipStream_t s) {
+  if (one_query)
+    hipLaunchKernelGGL((k_probe<W, 256>), dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_probe<W, 1024>), dim3(grid), dim3(1024), 0, s, a);
+}
+
 void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
   const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
   // w lanes share one key: each reads 8/w slots of the 128-B bucket, so a
   // wave's loads of one slot index cover 64/w buckets, each line coalesced
   switch (w) {
-    case 8: hipLaunchKernelGGL(k_probe<8>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_probe<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_probe<2>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_probe<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 8: probe_launch<8>(a, grid, a.nq == 1, s); break;
+    case 4: probe_launch<4>(a, grid, a.nq == 1, s); break;
+    case 2: probe_launch<2>(a, grid, a.nq == 1, s); break;
+    default: probe_launch<1>(a, grid, a.nq == 1, s); break;
   }
 }
 
@@ -635,6 +978,24 @@ void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
 
 void launch_scan_queries(const StepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
+}
+
+void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
+  const uint32_t nqb = (a.nq + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_query_sums, dim3(nqb + kPersistentGrid / 4), dim3(kBlock), 0, s, a, nqb);
+  hipLaunchKernelGGL(k_scan_queries_mb, dim3(nqb), dim3(kBlock), 0, s, a, nqb);
+}
+
+void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s) {
+  launch_scan_tiles(a, s);
+  launch_scan_rows(a, 0, s);
+  if (a.nq > uint32_t(kBlock) && a.qagg) {
+    launch_scan_queries_mb(a, s);  // output tiles are marked by its first launch
+    launch_gather_only(a, rows_per_wave, 0, s);
+  } else {
+    launch_scan_queries(a, s);
+    launch_gather(a, rows_per_wave, 0, s);
+  }
 }
 
 void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {

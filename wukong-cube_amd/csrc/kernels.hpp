@@ -28,11 +28,16 @@ struct DevSeg {
   uint64_t total_buckets;// main + extended: bound on chain links
   uint64_t edge_start;   // global edge offset of the segment (ptr.off - edge_start = local)
   uint64_t num_edges;
+  uint64_t nb_magic;     // bucket_magic(num_buckets)
   uint32_t kb_base;      // first entry in the key-block table
   uint32_t vb_base;      // first entry in the value-block table
   uint32_t resident;
+  uint32_t kphys;        // first physical key block when the segment's key blocks are
+                         // consecutive in the arena, else kNoPhys
+  uint32_t vphys;        // same for the value blocks
   uint32_t pad;
 };
+constexpr uint32_t kNoPhys = 0xFFFFFFFFu;
 
 // One query's work in one pattern step.
 struct StepDesc {
@@ -90,6 +95,7 @@ struct StepArgs {
   StepState* st;
   StepState* next_st;      // or null
   uint64_t* qres;          // [nq][2] rows and arena offset of each final table, or null
+  uint64_t* qagg;          // [ceil(nq / kBlock)][3] per-block query sums (multi-block query scan)
 };
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
@@ -102,6 +108,11 @@ constexpr int kOutTile = 4 * kBlock;            // output rows per gather tile (
 // unknown on the host: a fixed persistent grid is used.
 void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s);
 // scan = scan_tiles + scan_rows + scan_queries
+// scan_tiles, scan_rows, the query scan (multi-block for more than kBlock
+// queries, which also marks output tiles) and the gather of one step with no
+// host size hints: the batched and served paths.
+void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s);
+void launch_scan_queries_mb(const StepArgs& a, hipStream_t s);
 void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s);
 void launch_scan_tiles(const StepArgs& a, hipStream_t s);
 void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s);

@@ -137,8 +137,7 @@ struct Server {
       }
       const StepArgs args = make_args(dev, 0, n, tin, tout, dev->tcap, false);
       launch_probe(args, dev->opts.probe_width, R, dev->stream);
-      launch_scan(args, R, dev->stream);
-      launch_gather(args, dev->opts.rows_per_wave, 0, dev->stream);
+      launch_expand_tail(args, dev->opts.rows_per_wave, dev->stream);
       if (checksum_now && nfinal) {
         launch_fill_u64(d_sums, nfinal, 0, dev->stream);
         launch_checksum_final(dev->arena, dev->d_desc, d_idx, nfinal, dev->d_state, d_sums, dev->stream);

@@ -62,6 +62,30 @@ WK_HD uint64_t hash_u64(uint64_t key) {
   return key;
 }
 
+// High 64 bits of the 128-bit product a*b.
+WK_HD uint64_t mulhi_u64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return uint64_t((unsigned __int128)a * b >> 64);
+#endif
+}
+
+// Reciprocal for bucket_of: floor(2^64 / d) for d >= 2, 2^64 - 1 for d = 1.
+WK_HD uint64_t bucket_magic(uint64_t d) {
+  if (d <= 1) return ~uint64_t(0);
+  const uint64_t m = ~uint64_t(0) / d;
+  return (~uint64_t(0) % d == d - 1) ? m + 1 : m;  // d divides 2^64
+}
+
+// h mod d (d >= 1) without a 64-bit division, with m = bucket_magic(d)
+// (Barrett reduction).  m > 2^64/d - 1 gives h*m/2^64 > h/d - 1, so the
+// quotient estimate is q or q-1 and one conditional subtraction completes it.
+WK_HD uint64_t bucket_of(uint64_t h, uint64_t d, uint64_t m) {
+  const uint64_t r = h - mulhi_u64(h, m) * d;
+  return r >= d ? r - d : r;
+}
+
 // Hash of one history-table row; a table's checksum is the sum (mod 2^64) of
 // its row hashes, so it does not depend on row order.
 WK_HD uint64_t row_hash(const uint32_t* row, uint32_t cols) {
