@@ -231,7 +231,10 @@ typedef struct {
 
 /* Run nq queries together: each pattern step of all queries is one set of
  * launches over a concatenated table with per-query offsets, captured in a
- * hipGraph when opts allow.  Result tables stay in the batch arena. */
+ * hipGraph when opts allow.  Result tables stay in the batch arena, in query
+ * order; a query whose result table does not fit the rest of the arena
+ * reports WK_E_NOMEM_BUDGET and the others go on.  A step whose intermediate
+ * tables overflow is rerun in halves, down to single queries. */
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph,
                  wk_batch_result* out);
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems);

@@ -288,3 +288,26 @@ def test_known_to_known_on_long_edge_lists():
         exp += int(np.isin(ds, lists[int(h)]).sum())
     assert want.shape[0] == exp and exp > 4000
     dev.close()
+
+
+@pytest.mark.parametrize("extra", [3, 300])  # 300: the multi-block query scan
+def test_batch_drops_only_the_final_tables_that_do_not_fit(univ10_store, univ10_oracle, extra):
+    """Final tables past the result arena's capacity are dropped one query at a
+    time (WK_E_NOMEM_BUDGET) while the batch runs once: the queries before
+    them keep exact results and nothing is rerun."""
+    dev = wg.Device(univ10_store, table_budget=4 << 20)  # 149,796 elements per table
+    tcap = dev.stats()["table_capacity_elems"]
+    GS = Q.U("ub:GraduateStudent")
+    q = ([(GS, wg.TYPE_ID, -1, wg.IN)], 1)   # one pattern: 23,151 rows straight into the arena
+    want, _, _ = univ10_oracle.query(*q)
+    n_fit = tcap // want.size
+    qs = [q] * (n_fit + extra)
+    before = dev.stats()["queries_run"]
+    res = dev.batch(qs, use_graph=False)
+    assert dev.stats()["queries_run"] - before == len(qs)   # one pass, no halving
+    for i, (t, v2c, rc) in enumerate(res):
+        if i < n_fit:
+            assert rc == 0 and same_rows(t, want), i
+        else:
+            assert rc == wg.WK_E_NOMEM_BUDGET, (i, rc)
+    dev.close()

@@ -31,6 +31,7 @@ void launch_batch_steps(wk_dev* dev, uint32_t S, uint32_t nq, bool timed) {
 int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
               bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end) {
   const uint32_t nq = q1 - q0;
+  double t = host_now();
   *arena_end = arena_base;
   uint32_t S = 0;
   for (uint32_t q = q0; q < q1; q++) {
@@ -84,6 +85,7 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     nvalid += dev->h_desc[i].in_rows;
   }
   dev->h_rs[nq] = nvalid;
+  t = host_phase(dev, "host:batch_fill", t);
   for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
   dev->h_state[0].arena_base = arena_base;
   auto fail = [&](int code) {
@@ -117,12 +119,14 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   } else {
     launch_batch_steps(dev, S, nq, dev->opts.profile != 0);
   }
+  t = host_phase(dev, "host:batch_upload_launch", t);
   if (hip_failed(hipMemcpyAsync(dev->h_qres, dev->d_qres, size_t(nq) * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                 dev->stream)) ||
       hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
       hip_failed(hipStreamSynchronize(dev->stream)))
     return fail(WK_E_HIP);
   unpin_segments(dev, segs);
+  t = host_phase(dev, "host:batch_wait", t);
   bool overflow = false;
   for (uint32_t k = 0; k < S; k++) overflow |= dev->h_state[k].status != 0;
   if (overflow) {
@@ -130,7 +134,8 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
       status[q0] = WK_E_NOMEM_BUDGET;
       return WK_OK;
     }
-    // split the batch; halves reuse the tables and append to the arena
+    // an intermediate table overflowed: split the batch; halves reuse the
+    // tables and append to the arena
     const uint32_t mid = q0 + nq / 2;
     uint64_t end1 = arena_base;
     rc = run_batch(dev, plans, q0, mid, arena_base, use_graph, rows, offsets, status, &end1);
@@ -142,6 +147,11 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     const uint32_t q = q0 + i;
     const Plan& p = plans[q];
     if (p.status != WK_OK || p.steps.empty()) continue;
+    if (dev->h_qres[2 * size_t(i) + 1] == kNoSpace) {
+      // its final table did not fit the rest of the arena
+      status[q] = WK_E_NOMEM_BUDGET;
+      continue;
+    }
     rows[q] = dev->h_qres[2 * size_t(i)];
     offsets[q] = dev->h_qres[2 * size_t(i) + 1];
     end = std::max<uint64_t>(end, offsets[q] + rows[q] * p.cols);

@@ -528,6 +528,7 @@ int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems) {
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out) {
   if (!dev || (!qs && nq) || !out || !out->rows || !out->cols || !out->status || !out->offset)
     return WK_E_INVALID_ARG;
+  const double t0 = host_now();
   // plans keep their storage from call to call
   std::vector<Plan>& plans = dev->plan_cache;
   if (plans.size() < nq) plans.resize(nq);
@@ -540,6 +541,7 @@ int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph
         out->v2c[size_t(i) * WK_MAX_VARS + v] =
             (plans[i].status == WK_OK && v < qs[i].nvars) ? plans[i].v2c[size_t(v)] : -1;
   }
+  host_phase(dev, "host:batch_compile", t0);
   uint64_t end = 0;
   int rc = run_batch(dev, plans, 0, nq, 0, use_graph != 0, out->rows, out->offset, out->status, &end);
   dev->batch_used = rc ? 0 : end;

@@ -4,6 +4,7 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -354,6 +355,20 @@ void timed_end(wk_dev* dev, hipEvent_t a, const char* name, double bytes) {
   }
   (void)hipEventRecord(b, dev->stream);
   dev->pending.push_back(PendingTiming{name, a, b, bytes});
+}
+
+double host_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+double host_phase(wk_dev* dev, const char* name, double t0) {
+  const double t = host_now();
+  if (dev->opts.profile) {
+    KernelStat& k = dev->kstats[name];
+    k.launches++;
+    k.ms += (t - t0) * 1e3;
+  }
+  return t;
 }
 
 int flush_timings(wk_dev* dev) {

@@ -161,6 +161,10 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
 // Timing helpers (active when opts.profile).
 void timed_begin(wk_dev* dev, hipEvent_t* ev);
 void timed_end(wk_dev* dev, hipEvent_t ev, const char* name, double bytes);
+// Host-side phase timing (active when opts.profile): adds the milliseconds
+// since `t0` (steady clock, seconds) to the stat `name` and returns now.
+double host_phase(wk_dev* dev, const char* name, double t0);
+double host_now();
 int flush_timings(wk_dev* dev);
 
 // Run one step of a single query on the host-synchronous path; writes the

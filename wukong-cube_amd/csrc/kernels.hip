@@ -144,46 +144,82 @@ __device__ __forceinline__ uint64_t block_sum(uint64_t v) {
 // Cluster-chained probe of one 128-byte bucket per hop by W cooperating
 // lanes (KVStore::get_slot_local, core/store/kvstore.hpp:292-314): slots 0..6
 // are compared with the key, slot 7 links to the next bucket or is empty.
+// One bucket of the cluster chain in two halves, so that the loads of
+// several buckets can be issued before any of them is compared: load_bucket reads the
+// lane's 8/W whole 16-B slots (key and ptr in one load each); scan_bucket
+// compares slots 0..6 with the key and takes slot 7 as the link to the next
+// bucket (0: end of chain).  With W > 1 the group's result is combined, so
+// hit, ptr and link are the same in all W lanes.
 template <int W>
-__device__ __forceinline__ bool probe(const StepArgs& a, const DevSeg& s, uint64_t key, uint64_t& ptr) {
+__device__ __forceinline__ void load_bucket(const StepArgs& a, const DevSeg& s, uint64_t b,
+                                            uint4 (&v)[ASSOCIATIVITY / W]) {
+  const int gl = threadIdx.x & (W - 1);
+  const uint4* bk = reinterpret_cast<const uint4*>(bucket_addr(a, s, b));
+#pragma unroll
+  for (int k = 0; k < ASSOCIATIVITY / W; ++k) v[k] = bk[gl + k * W];
+}
+
+template <int W>
+__device__ __forceinline__ void scan_bucket(const uint4 (&v)[ASSOCIATIVITY / W], uint64_t key, bool& hit,
+                                            uint64_t& ptr, uint64_t& link) {
   const int lane = threadIdx.x & (kWave - 1);
   const int gl = lane & (W - 1);
   const int gbase = lane & ~(W - 1);
-  uint64_t b = bucket_of(hash_u64(key), s.num_buckets, s.nb_magic);
-  for (uint64_t hop = 0; hop < s.total_buckets; ++hop) {
-    const uint4* bk = reinterpret_cast<const uint4*>(bucket_addr(a, s, b));
-    bool hit = false;
-    uint64_t hptr = 0, link = 0;
+  bool h = false;
+  uint64_t hptr = 0, l = 0;
 #pragma unroll
-    for (int k = 0; k < ASSOCIATIVITY / W; ++k) {
-      const int si = gl + k * W;
-      const uint4 v = bk[si];  // whole 16-B slot: key and ptr in one load
-      const uint64_t sk = uint64_t(v.x) | (uint64_t(v.y) << 32);
-      const uint64_t sp = uint64_t(v.z) | (uint64_t(v.w) << 32);
-      const bool m = si < ASSOCIATIVITY - 1 && sk == key;
-      hit = hit || m;
-      hptr = m ? sp : hptr;
-      link = (si == ASSOCIATIVITY - 1) ? sk : link;
-    }
-    if constexpr (W == 1) {
-      if (hit) {
-        ptr = hptr;
-        return true;
-      }
-    } else {
-      const uint64_t m = (__ballot(hit) >> gbase) & ((uint64_t(1) << W) - 1);
-      if (m) {
-        const int src = gbase + __ffsll((long long)m) - 1;
-        ptr = __shfl(hptr, src, kWave);
-        return true;
-      }
-      link = __shfl(link, gbase + ((ASSOCIATIVITY - 1) & (W - 1)), kWave);
-    }
-    if (link == 0) return false;
-    b = key_vid(link);
+  for (int k = 0; k < ASSOCIATIVITY / W; ++k) {
+    const int si = gl + k * W;
+    const uint64_t sk = uint64_t(v[k].x) | (uint64_t(v[k].y) << 32);
+    const uint64_t sp = uint64_t(v[k].z) | (uint64_t(v[k].w) << 32);
+    const bool m = si < ASSOCIATIVITY - 1 && sk == key;
+    h = h || m;
+    hptr = m ? sp : hptr;
+    l = (si == ASSOCIATIVITY - 1) ? sk : l;
+  }
+  if constexpr (W == 1) {
+    hit = h;
+    ptr = hptr;
+    link = l;
+  } else {
+    const uint64_t m = (__ballot(h) >> gbase) & ((uint64_t(1) << W) - 1);
+    hit = m != 0;
+    ptr = __shfl(hptr, gbase + (m ? __ffsll((long long)m) - 1 : 0), kWave);
+    link = __shfl(l, gbase + ((ASSOCIATIVITY - 1) & (W - 1)), kWave);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void probe_hop(const StepArgs& a, const DevSeg& s, uint64_t key, uint64_t b, bool& hit,
+                                          uint64_t& ptr, uint64_t& link) {
+  uint4 v[ASSOCIATIVITY / W];
+  load_bucket<W>(a, s, b, v);
+  scan_bucket<W>(v, key, hit, ptr, link);
+}
+
+// The rest of a cluster chain after a first bucket that missed with `link`.
+template <int W>
+__device__ __forceinline__ bool probe_chain(const StepArgs& a, const DevSeg& s, uint64_t key, uint64_t link,
+                                            uint64_t& ptr) {
+  for (uint64_t hop = 1; hop < s.total_buckets && link != 0; ++hop) {
+    const uint64_t b = key_vid(link);
     if (b >= s.total_buckets) return false;  // malformed link: treat as absent
+    bool hit;
+    probe_hop<W>(a, s, key, b, hit, ptr, link);
+    if (hit) return true;
   }
   return false;
+}
+
+// Cluster-chained probe of one 128-byte bucket per hop by W cooperating
+// lanes (KVStore::get_slot_local, core/store/kvstore.hpp:292-314).
+template <int W>
+__device__ __forceinline__ bool probe(const StepArgs& a, const DevSeg& s, uint64_t key, uint64_t& ptr) {
+  bool hit;
+  uint64_t link;
+  probe_hop<W>(a, s, key, bucket_of(hash_u64(key), s.num_buckets, s.nb_magic), hit, ptr, link);
+  if (hit) return true;
+  return probe_chain<W>(a, s, key, link, ptr);
 }
 
 // Membership of x in the sorted, duplicate-free list of n edges read by
@@ -291,6 +327,49 @@ __device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d,
   }
 }
 
+// Two rows of one query at once: both keys, then both first buckets are
+// loaded before either is compared, so each lane keeps two dependent load
+// chains in flight.  Chains past the first bucket and the membership tests
+// of filters follow row by row.  `s` must be resident.
+template <int W>
+__device__ __forceinline__ void finish_row(const StepArgs& a, const ProbeDesc& d, const DevSeg& s,
+                                           const uint32_t* row, bool hit, uint64_t ptr, uint32_t& c, uint32_t& eo) {
+  if (!hit) return;
+  const uint64_t sz = ptr_size(ptr);
+  const uint64_t off = ptr_off(ptr) - s.edge_start;
+  if (d.op == OP_EXPAND) {
+    c = uint32_t(sz);
+    eo = uint32_t(off);
+  } else {
+    uint32_t target = d.kend;
+    if (d.op == OP_FILTER_KNOWN) target = row[d.ec];
+    c = contains(a, s, off, sz, target) ? 1u : 0u;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void probe_row2(const StepArgs& a, const ProbeDesc& d, const DevSeg& s,
+                                           const uint32_t* row0, const uint32_t* row1, uint32_t& c0, uint32_t& eo0,
+                                           uint32_t& c1, uint32_t& eo1) {
+  const bool konst = d.key_mode == KEY_CONST, tix = d.key_mode == KEY_TYPE_INDEX;
+  const uint32_t kv0 = konst ? d.kstart : row0[d.sc];
+  const uint32_t kv1 = konst ? d.kstart : row1[d.sc];
+  const uint64_t key0 = tix ? make_key(0, kv0, DIR_IN) : make_key(kv0, d.pid, d.dir);
+  const uint64_t key1 = tix ? make_key(0, kv1, DIR_IN) : make_key(kv1, d.pid, d.dir);
+  uint4 v0[ASSOCIATIVITY / W], v1[ASSOCIATIVITY / W];
+  load_bucket<W>(a, s, bucket_of(hash_u64(key0), s.num_buckets, s.nb_magic), v0);
+  load_bucket<W>(a, s, bucket_of(hash_u64(key1), s.num_buckets, s.nb_magic), v1);
+  bool h0, h1;
+  uint64_t p0, p1, l0, l1;
+  scan_bucket<W>(v0, key0, h0, p0, l0);
+  scan_bucket<W>(v1, key1, h1, p1, l1);
+  if (!h0 && l0 != 0) h0 = probe_chain<W>(a, s, key0, l0, p0);
+  if (!h1 && l1 != 0) h1 = probe_chain<W>(a, s, key1, l1, p1);
+  // KEY_TYPE_INDEX reads [0|t|IN]: only type/predicate ids fit the 17-bit pid field
+  finish_row<W>(a, d, s, row0, h0 && (!tix || kv0 < (1u << NBITS_IDX)), p0, c0, eo0);
+  finish_row<W>(a, d, s, row1, h1 && (!tix || kv1 < (1u << NBITS_IDX)), p1, c1, eo1);
+}
+
 // One block of NT threads per tile of kTile rows, in kTile / (NT / W)
 // passes.  A tile inside one query reads the descriptor and segment once,
 // into scalar registers; a tile spanning several queries stages their row
@@ -323,15 +402,41 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
       const DevSeg s = a.segs[live ? d.seg : 0];
       const uint32_t* const tbase = a.tin + a.desc[q].in_off;
       const uint64_t r0 = a.row_start[q];
-      for (int pass = 0; pass < kPasses; ++pass) {
-        const uint64_t g = tlo + uint64_t(pass) * G + grp;
-        if (g < thi) {
-          uint32_t c = 0, eo = 0;
-          if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
-          if (gl == 0) {
-            a.cnt[g] = c;
-            a.eoff[g] = eo;
-            tsum += c;
+      if constexpr (kPasses % 2 == 0) {
+        // rows g and g + G of each pair of passes together
+        const bool res = live && s.resident != 0;
+        for (int pass = 0; pass < kPasses; pass += 2) {
+          const uint64_t g = tlo + uint64_t(pass) * G + grp;
+          if (g < thi) {
+            const uint64_t g1 = g + G;
+            const bool has1 = g1 < thi;
+            const uint64_t gr1 = has1 ? g1 : g;  // a lone last row probes itself twice
+            uint32_t c = 0, eo = 0, c1 = 0, eo1 = 0;
+            if (res)
+              probe_row2<W>(a, d, s, tbase + (g - r0) * d.in_cols, tbase + (gr1 - r0) * d.in_cols, c, eo, c1, eo1);
+            if (gl == 0) {
+              a.cnt[g] = c;
+              a.eoff[g] = eo;
+              tsum += c;
+              if (has1) {
+                a.cnt[g1] = c1;
+                a.eoff[g1] = eo1;
+                tsum += c1;
+              }
+            }
+          }
+        }
+      } else {
+        for (int pass = 0; pass < kPasses; ++pass) {
+          const uint64_t g = tlo + uint64_t(pass) * G + grp;
+          if (g < thi) {
+            uint32_t c = 0, eo = 0;
+            if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
+            if (gl == 0) {
+              a.cnt[g] = c;
+              a.eoff[g] = eo;
+              tsum += c;
+            }
           }
         }
       }
@@ -545,10 +650,13 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
       a.next_row_start[q] = xn;
     }
     if (fin) {
-      d.out_off = xa;
+      // a final table that does not fit the arena is dropped alone
+      const bool fits = xa + el <= a.arena_cap;
+      d.out_off = fits ? xa : kNoSpace;
+      if (!fits) d.out_rows = 0;
       if (a.qres) {
-        a.qres[2 * uint64_t(q)] = rows;
-        a.qres[2 * uint64_t(q) + 1] = xa;
+        a.qres[2 * uint64_t(q)] = fits ? rows : 0;
+        a.qres[2 * uint64_t(q) + 1] = fits ? xa : kNoSpace;
       }
       xa += el;
     } else {
@@ -559,9 +667,7 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
   }
   if (threadIdx.x == 0) {
     const uint64_t carry_t = tbase + tt, carry_a = abase + ta;
-    uint32_t status = 0;
-    if (carry_t > a.tcap) status = 1;
-    if (carry_a > a.arena_cap) status = 2;
+    const uint32_t status = carry_t > a.tcap ? 1 : 0;
     a.st->out_elems = tt;
     a.st->status = status;
     if (a.next) {
@@ -684,24 +790,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
   const uint64_t xn = block_excl_scan<kBlock>(fin ? 0 : rows, nullptr) + v[2];
   if (q < a.nq) {
     StepDesc& d = a.desc[q];
-    d.out_rows = rows;
-    d.out_off = fin ? xa : xt;
+    // a final table that does not fit the arena is dropped alone
+    const bool fits = !fin || xa + el <= a.arena_cap;
+    d.out_rows = fits ? rows : 0;
+    d.out_off = fin ? (fits ? xa : kNoSpace) : xt;
     if (a.next) {
       a.next[q].in_rows = fin ? 0 : rows;
       a.next[q].in_off = xt;
       a.next_row_start[q] = xn;
     }
     if (fin && a.qres) {
-      a.qres[2 * uint64_t(q)] = rows;
-      a.qres[2 * uint64_t(q) + 1] = xa;
+      a.qres[2 * uint64_t(q)] = fits ? rows : 0;
+      a.qres[2 * uint64_t(q) + 1] = fits ? xa : kNoSpace;
     }
   }
   if (blockIdx.x == nqb - 1 && threadIdx.x == 0) {
     const uint64_t tt = v[3], ta = v[4], tn = v[5];
     const uint64_t carry_t = tbase + tt, carry_a = abase + ta;
-    uint32_t status = 0;
-    if (carry_t > a.tcap) status = 1;
-    if (carry_a > a.arena_cap) status = 2;
+    const uint32_t status = carry_t > a.tcap ? 1 : 0;
     a.st->out_elems = tt;
     a.st->status = status;
     if (a.next) {
@@ -723,6 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
 __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t r0, uint64_t p0, uint64_t g,
                                          uint64_t j, uint64_t pg, uint32_t eo) {
   const StepDesc& d = a.desc[q];
+  if (d.out_off == kNoSpace) return;  // final table dropped: no room in the arena
   const uint64_t orow = j - p0;
   uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
   const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
@@ -795,7 +902,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       s_qp0[threadIdx.x] = a.pfx[r0 < R ? r0 : R];
     }
     __syncthreads();
-    if (staged && qhi - qlo == 1) {
+    if (staged && qhi - qlo == 1 && a.desc[qlo].out_off == kNoSpace) {
+      // the tile's one query lost its final table (no room in the arena)
+    } else if (staged && qhi - qlo == 1) {
       // one query owns the whole tile: its descriptor, row start and segment
       // are loaded once, into scalar registers
       const uint32_t q = __builtin_amdgcn_readfirstlane(qlo);

@@ -38,6 +38,10 @@ struct DevSeg {
   uint32_t pad;
 };
 constexpr uint32_t kNoPhys = 0xFFFFFFFFu;
+// StepDesc.out_off (and the arena offset in qres) of a final table that did
+// not fit the result arena: the query reports WK_E_NOMEM_BUDGET, the others
+// of the step go on.
+constexpr uint64_t kNoSpace = ~uint64_t(0);
 
 // One query's work in one pattern step.
 struct StepDesc {

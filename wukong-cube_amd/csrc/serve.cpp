@@ -157,9 +157,14 @@ struct Server {
         for (uint32_t i = 0; i < n; i++) res[a + i] = dev->h_desc[i];
         *used_t = tout_base + dev->h_state[0].out_elems;
         uint64_t ua = arena_base;
-        for (uint32_t i = 0; i < n; i++)
-          if (dev->h_desc[i].final_step)
-            ua = std::max<uint64_t>(ua, dev->h_desc[i].out_off + dev->h_desc[i].out_rows * dev->h_desc[i].out_cols);
+        for (uint32_t i = 0; i < n; i++) {
+          if (!dev->h_desc[i].final_step) continue;
+          if (dev->h_desc[i].out_off == kNoSpace) {
+            failed.push_back(int(a + i));  // its final table did not fit the arena
+            continue;
+          }
+          ua = std::max<uint64_t>(ua, dev->h_desc[i].out_off + dev->h_desc[i].out_rows * dev->h_desc[i].out_cols);
+        }
         *used_a = ua;
         if (checksum_now)
           for (uint32_t f = 0; f < nfinal; f++) res[a + h_idx[f]].in_off = h_sums[f];  // carry checksum
