@@ -281,16 +281,19 @@ def main():
         for mode, rate in plan:
             n = min(len(qs), int(rate * 2.0) + 256)
             arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
+            dev.reset_kernel_stats()
             r = dev.serve(qs[:n], arr, max_inflight=4096, per_query=(mode == "per_query"))
+            phases = {k[6:]: {"count": v["launches"], "ms": round(v["total_ms"], 3)}
+                      for k, v in dev.kernel_stats().items() if k.startswith("serve:")}
             lat = r["latency_us"]
             ok = r["status"] == 0
             serve[mode] = {"offered_qps": rate, "queries": n, "achieved_qps": n / r["wall_s"],
                            "p50_us": float(np.percentile(lat[ok], 50)) if ok.any() else None,
                            "p99_us": float(np.percentile(lat[ok], 99)) if ok.any() else None,
-                           "ok": int(ok.sum()), "iterations": int(r["batches"]), "_ck": r["checksum"],
-                           "_st": r["status"], "_n": n}
-            log("serve %s @ %.0f q/s offered: %.0f q/s, p50 %s us, p99 %s us, %d ok" % (
-                mode, rate, n / r["wall_s"], serve[mode]["p50_us"], serve[mode]["p99_us"], ok.sum()))
+                           "ok": int(ok.sum()), "iterations": int(r["batches"]), "phases": phases,
+                           "_ck": r["checksum"], "_st": r["status"], "_n": n}
+            log("serve %s @ %.0f q/s offered: %.0f q/s, p50 %s us, p99 %s us, %d ok; %s" % (
+                mode, rate, n / r["wall_s"], serve[mode]["p50_us"], serve[mode]["p99_us"], ok.sum(), phases))
 
     # ------------------------------------------------------ CPU baseline + gate
     cpu = None

@@ -834,11 +834,20 @@ __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t
   uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
   const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
   const uint32_t ic = d.in_cols;
-  for (uint32_t c = 0; c < ic; ++c) dst[c] = src[c];
+  const bool ck = a.ck && d.final_step;
+  uint64_t h = row_hash_init(d.out_cols);
+  for (uint32_t c = 0; c < ic; ++c) {
+    const uint32_t x = src[c];
+    dst[c] = x;
+    if (ck) h = row_hash_step(h, x, c);
+  }
   if (d.op == OP_EXPAND) {
     const DevSeg s = a.segs[d.seg];
-    dst[ic] = edge_at(a, s, uint64_t(eo) + (j - pg));
+    const uint32_t x = edge_at(a, s, uint64_t(eo) + (j - pg));
+    dst[ic] = x;
+    if (ck) h = row_hash_step(h, x, ic);
   }
+  if (ck) atomicAdd(&a.ck[q], (unsigned long long)h);
 }
 
 __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
@@ -916,6 +925,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       uint32_t* const dbase = (d.final_step ? a.arena : a.tout) + d.out_off;
       const uint32_t* const sbase = a.tin + d.in_off;
       const DevSeg sg = a.segs[expand ? d.seg : 0];
+      const bool ck = a.ck && d.final_step;
+      uint64_t hsum = 0;  // row hashes of this thread's final rows
       uint32_t l = 0;  // source rows only move forward as j grows
 #pragma unroll
       for (int k = 0; k < kOutTile / kBlock; ++k) {
@@ -930,8 +941,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
         const uint64_t g = lo + l;
         uint32_t* dst = dbase + (j - p0) * oc;
         const uint32_t* src = sbase + (g - r0) * ic;
-        for (uint32_t c = 0; c < ic; ++c) dst[c] = src[c];
-        if (expand) dst[ic] = edge_at(a, sg, uint64_t(s_eoff[l]) + (j - s_pfx[l]));
+        uint64_t rh = row_hash_init(oc);
+        for (uint32_t c = 0; c < ic; ++c) {
+          const uint32_t x = src[c];
+          dst[c] = x;
+          if (ck) rh = row_hash_step(rh, x, c);
+        }
+        if (expand) {
+          const uint32_t x = edge_at(a, sg, uint64_t(s_eoff[l]) + (j - s_pfx[l]));
+          dst[ic] = x;
+          if (ck) rh = row_hash_step(rh, x, ic);
+        }
+        if (ck) hsum += rh;
+      }
+      if (ck) {
+        hsum = block_sum<kBlock>(hsum);
+        if (threadIdx.x == 0 && hsum) atomicAdd(&a.ck[q], (unsigned long long)hsum);
       }
     } else {
 #pragma unroll
@@ -1026,22 +1051,6 @@ __global__ __launch_bounds__(kBlock) void k_checksum(const uint32_t* base, const
     if (qa + threadIdx.x < n && acc[threadIdx.x]) atomicAdd(&sums[qa + threadIdx.x], acc[threadIdx.x]);
     __syncthreads();
   }
-}
-
-// Checksums of the final tables of one step: table f = desc[idx[f]] in the
-// arena, handled by `chunks` blocks.
-__global__ __launch_bounds__(kBlock) void k_checksum_final(const uint32_t* arena, const StepDesc* desc,
-                                                           const uint32_t* idx, uint32_t n, uint32_t chunks,
-                                                           const StepState* st, unsigned long long* sums) {
-  if (st->status) return;
-  const uint32_t f = blockIdx.x / chunks, ch = blockIdx.x % chunks;
-  if (f >= n) return;
-  const StepDesc& d = desc[idx[f]];
-  uint64_t s = 0;
-  for (uint64_t r = uint64_t(ch) * kBlock + threadIdx.x; r < d.out_rows; r += uint64_t(chunks) * kBlock)
-    s += row_hash(arena + d.out_off + r * d.out_cols, d.out_cols);
-  s = block_sum<kBlock>(s);
-  if (threadIdx.x == 0 && s) atomicAdd(&sums[f], (unsigned long long)s);
 }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
@@ -1148,13 +1157,6 @@ void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t
                      base, d, n, total_rows, reinterpret_cast<unsigned long long*>(sums));
 }
 
-void launch_checksum_final(const uint32_t* arena, const StepDesc* desc, const uint32_t* idx, uint32_t n,
-                           const StepState* st, uint64_t* sums, hipStream_t s) {
-  if (n == 0) return;
-  const uint32_t chunks = 16;
-  hipLaunchKernelGGL(k_checksum_final, dim3(n * chunks), dim3(kBlock), 0, s, arena, desc, idx, n, chunks, st,
-                     reinterpret_cast<unsigned long long*>(sums));
-}
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((n + 255) / 256, 4096)), dim3(256), 0, s, p, n, v);

@@ -100,6 +100,7 @@ struct StepArgs {
   StepState* next_st;      // or null
   uint64_t* qres;          // [nq][2] rows and arena offset of each final table, or null
   uint64_t* qagg;          // [ceil(nq / kBlock)][3] per-block query sums (multi-block query scan)
+  unsigned long long* ck;  // [nq] sums of the row hashes of final tables (zeroed before), or null
 };
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
@@ -140,7 +141,6 @@ struct CkDesc {
 void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t total_rows, uint64_t* sums,
                      hipStream_t s);
 // sums[f] += checksum of the final table desc[idx[f]] in the arena.
-void launch_checksum_final(const uint32_t* arena, const StepDesc* desc, const uint32_t* idx, uint32_t n,
-                           const StepState* st, uint64_t* sums, hipStream_t s);
+
 
 }  // namespace wk

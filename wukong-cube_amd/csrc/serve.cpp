@@ -54,29 +54,21 @@ struct Server {
   wk_dev* dev;
   const std::vector<Plan>& plans;
   std::vector<QState>& qs;
-  uint32_t* d_idx = nullptr;
   uint64_t* d_sums = nullptr;
-  uint32_t* h_idx = nullptr;
   uint64_t* h_sums = nullptr;
   uint32_t cap = 0;
 
   Server(wk_dev* d, const std::vector<Plan>& p, std::vector<QState>& s) : dev(d), plans(p), qs(s) {}
   ~Server() {
-    if (d_idx) (void)hipFree(d_idx);
     if (d_sums) (void)hipFree(d_sums);
-    if (h_idx) (void)hipHostFree(h_idx);
     if (h_sums) (void)hipHostFree(h_sums);
   }
   int reserve(uint32_t n) {
     if (n <= cap) return WK_OK;
-    if (d_idx) (void)hipFree(d_idx);
     if (d_sums) (void)hipFree(d_sums);
-    if (h_idx) (void)hipHostFree(h_idx);
     if (h_sums) (void)hipHostFree(h_sums);
     cap = std::max<uint32_t>(n, 1024);
-    WK_HIP(hipMalloc(&d_idx, cap * sizeof(uint32_t)));
     WK_HIP(hipMalloc(&d_sums, cap * sizeof(uint64_t)));
-    WK_HIP(hipHostMalloc(&h_idx, cap * sizeof(uint32_t), hipHostMallocDefault));
     WK_HIP(hipHostMalloc(&h_sums, cap * sizeof(uint64_t), hipHostMallocDefault));
     return WK_OK;
   }
@@ -97,7 +89,6 @@ struct Server {
     if (rc) return rc;
     std::vector<int> segs;
     uint64_t R = 0;
-    uint32_t nfinal = 0;
     for (uint32_t i = 0; i < n; i++) {
       const uint32_t q = F[a + i];
       const Plan& p = plans[q];
@@ -108,7 +99,6 @@ struct Server {
       dev->h_desc[i] = d;
       dev->h_rs[i] = R;
       R += d.in_rows;
-      if (d.final_step) h_idx[nfinal++] = i;
       if (d.seg >= 0 && std::find(segs.begin(), segs.end(), d.seg) == segs.end()) segs.push_back(d.seg);
     }
     dev->h_rs[n] = R;
@@ -129,22 +119,25 @@ struct Server {
       uint32_t* tout = dev->tbuf[cur ^ 1];
       if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, n * sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream)) ||
           hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
-          hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream)) ||
-          hip_failed(hipMemcpyAsync(d_idx, h_idx, std::max<uint32_t>(nfinal, 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
-                         dev->stream))) {
+          hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
       }
-      const StepArgs args = make_args(dev, 0, n, tin, tout, dev->tcap, false);
+      StepArgs args = make_args(dev, 0, n, tin, tout, dev->tcap, false);
+      // the gather sums the row hashes of each final table as it writes it
+      if (checksum_now) {
+        launch_fill_u64(d_sums, n, 0, dev->stream);
+        args.ck = reinterpret_cast<unsigned long long*>(d_sums);
+      }
+      hipEvent_t ev = nullptr;
+      timed_begin(dev, &ev);
       launch_probe(args, dev->opts.probe_width, R, dev->stream);
       launch_expand_tail(args, dev->opts.rows_per_wave, dev->stream);
-      if (checksum_now && nfinal) {
-        launch_fill_u64(d_sums, nfinal, 0, dev->stream);
-        launch_checksum_final(dev->arena, dev->d_desc, d_idx, nfinal, dev->d_state, d_sums, dev->stream);
-        if (hip_failed(hipMemcpyAsync(h_sums, d_sums, nfinal * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
-          unpin_segments(dev, segs);
-          return WK_E_HIP;
-        }
+      timed_end(dev, ev, "serve:step_kernels", 0);
+      if (checksum_now &&
+          hip_failed(hipMemcpyAsync(h_sums, d_sums, n * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
+        unpin_segments(dev, segs);
+        return WK_E_HIP;
       }
       if (hip_failed(hipMemcpyAsync(dev->h_desc, dev->d_desc, n * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream)) ||
           hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
@@ -167,7 +160,8 @@ struct Server {
         }
         *used_a = ua;
         if (checksum_now)
-          for (uint32_t f = 0; f < nfinal; f++) res[a + h_idx[f]].in_off = h_sums[f];  // carry checksum
+          for (uint32_t i = 0; i < n; i++)
+            if (dev->h_desc[i].final_step) res[a + i].in_off = h_sums[i];  // carry the checksum
         return WK_OK;
       }
       if (n == 1) {
@@ -176,6 +170,7 @@ struct Server {
       }
     }
   split : {
+    if (dev->opts.profile) dev->kstats["serve:splits"].launches++;
     const size_t m = a + n / 2;
     uint64_t t1 = 0, a1 = 0;
     rc = step_range(F, a, m, cur, tout_base, arena_base, &t1, &a1, res, failed, checksum_now);
@@ -318,8 +313,11 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     res.assign(inflight.size(), StepDesc{});
     failed.clear();
     uint64_t used_t = 0, used_a = 0;
+    const double t_it = host_now();
     rc = srv.step_range(inflight, 0, inflight.size(), cur, 0, 0, &used_t, &used_a, res, failed, true);
     if (rc) break;
+    host_phase(dev, "serve:iteration", t_it);
+    if (dev->opts.profile && (rc = flush_timings(dev)) != WK_OK) break;
     const double t = now_s();
     std::vector<uint32_t> still;
     for (size_t i = 0; i < inflight.size(); i++) {

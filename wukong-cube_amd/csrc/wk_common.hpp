@@ -88,9 +88,13 @@ WK_HD uint64_t bucket_of(uint64_t h, uint64_t d, uint64_t m) {
 
 // Hash of one history-table row; a table's checksum is the sum (mod 2^64) of
 // its row hashes, so it does not depend on row order.
+WK_HD uint64_t row_hash_init(uint32_t cols) { return 0x9E3779B97F4A7C15ull ^ cols; }
+WK_HD uint64_t row_hash_step(uint64_t h, uint32_t x, uint32_t c) {
+  return hash_u64(h ^ (uint64_t(x) + (uint64_t(c) << 32)));
+}
 WK_HD uint64_t row_hash(const uint32_t* row, uint32_t cols) {
-  uint64_t h = 0x9E3779B97F4A7C15ull ^ cols;
-  for (uint32_t c = 0; c < cols; c++) h = hash_u64(h ^ (uint64_t(row[c]) + (uint64_t(c) << 32)));
+  uint64_t h = row_hash_init(cols);
+  for (uint32_t c = 0; c < cols; c++) h = row_hash_step(h, row[c], c);
   return h;
 }
 
