@@ -327,6 +327,29 @@ __device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d,
   }
 }
 
+// Exclusive scan of the per-tile edge counts by one block of NT threads;
+// sets the step's total output rows.
+template <int NT>
+__device__ __forceinline__ void scan_tiles_block(const StepArgs& a) {
+  const uint64_t R = step_rows(a);
+  uint64_t nt = (R + kTile - 1) / kTile;
+  if (nt > a.tile_cap) nt = a.tile_cap;
+  const uint64_t per = (nt + NT - 1) / NT;
+  uint64_t lo = uint64_t(threadIdx.x) * per;
+  if (lo > nt) lo = nt;
+  const uint64_t hi = (lo + per < nt) ? lo + per : nt;
+  uint64_t s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += a.tile_sum[i];
+  uint64_t total = 0;
+  uint64_t ex = block_excl_scan<NT>(s, &total);
+  for (uint64_t i = lo; i < hi; ++i) {
+    const uint64_t v = a.tile_sum[i];
+    a.tile_sum[i] = ex;
+    ex += v;
+  }
+  if (threadIdx.x == 0) a.st->total_out_rows = total;
+}
+
 // Two rows of one query at once: both keys, then both first buckets are
 // loaded before either is compared, so each lane keeps two dependent load
 // chains in flight.  Chains past the first bucket and the membership tests
@@ -482,23 +505,7 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
 
 __global__ __launch_bounds__(1024) void k_scan_tiles(StepArgs a) {
   if (a.st->status) return;
-  const uint64_t R = step_rows(a);
-  uint64_t nt = (R + kTile - 1) / kTile;
-  if (nt > a.tile_cap) nt = a.tile_cap;
-  const uint64_t per = (nt + 1023) / 1024;
-  uint64_t lo = uint64_t(threadIdx.x) * per;
-  if (lo > nt) lo = nt;
-  const uint64_t hi = (lo + per < nt) ? lo + per : nt;
-  uint64_t s = 0;
-  for (uint64_t i = lo; i < hi; ++i) s += a.tile_sum[i];
-  uint64_t total = 0;
-  uint64_t ex = block_excl_scan<1024>(s, &total);
-  for (uint64_t i = lo; i < hi; ++i) {
-    const uint64_t v = a.tile_sum[i];
-    a.tile_sum[i] = ex;
-    ex += v;
-  }
-  if (threadIdx.x == 0) a.st->total_out_rows = total;
+  scan_tiles_block<1024>(a);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {

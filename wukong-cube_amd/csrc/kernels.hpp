@@ -66,7 +66,7 @@ struct StepState {
   uint64_t tout_base;       // first element of the ping-pong out table to use
   uint64_t out_elems;       // elements written to the out table (set by scan_queries)
   uint64_t arena_base;      // first free element of the result arena
-  uint32_t status;          // 0 ok, 1 table overflow, 2 arena overflow
+  uint32_t status;          // 0 ok, 1 table overflow
   uint32_t pad;
 };
 
