@@ -833,10 +833,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
 // Writes output row j of query q: input row g (whose first output is row
 // pg of the step) and, for an expansion, edge eo + (j - pg) of its list.
 // r0 is the query's first input row and p0 = pfx[r0] its first output row.
-__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t r0, uint64_t p0, uint64_t g,
-                                         uint64_t j, uint64_t pg, uint32_t eo) {
+// Returns the row's hash when the step keeps checksums of final tables
+// (a.ck) and this is one, else 0; the caller adds it to the query's sum.
+__device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t r0, uint64_t p0,
+                                             uint64_t g, uint64_t j, uint64_t pg, uint32_t eo) {
   const StepDesc& d = a.desc[q];
-  if (d.out_off == kNoSpace) return;  // final table dropped: no room in the arena
+  if (d.out_off == kNoSpace) return 0;  // final table dropped: no room in the arena
   const uint64_t orow = j - p0;
   uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
   const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
@@ -854,17 +856,21 @@ __device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t
     dst[ic] = x;
     if (ck) h = row_hash_step(h, x, ic);
   }
-  if (ck) atomicAdd(&a.ck[q], (unsigned long long)h);
+  return ck ? h : 0;
 }
 
-__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
-                                         uint32_t eo) {
+__device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
+                                             uint32_t eo) {
   const uint64_t r0 = a.row_start[q];
-  emit_row(a, q, r0, a.pfx[r0], g, j, pg, eo);
+  return emit_row(a, q, r0, a.pfx[r0], g, j, pg, eo);
 }
 
-__device__ __forceinline__ void emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
-  emit_row(a, q, g, j, a.pfx[g], a.eoff[g]);
+__device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
+  return emit_row(a, q, g, j, a.pfx[g], a.eoff[g]);
+}
+
+__device__ __forceinline__ void add_checksum(const StepArgs& a, uint32_t q, uint64_t h) {
+  if (h) atomicAdd(&a.ck[q], (unsigned long long)h);
 }
 
 // Output-balanced: a block writes one tile of kOutTile consecutive output
@@ -882,6 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   __shared__ uint32_t s_eoff[kStage];
   __shared__ uint64_t s_qr0[kQStage];  // first input row of each query of the tile
   __shared__ uint64_t s_qp0[kQStage];  // and its first output row
+  __shared__ unsigned long long s_ck[kQStage];  // and its checksum share from this tile
   const uint64_t R = step_rows(a);
   const uint64_t total = a.st->total_out_rows;
   uint64_t nt = (total + kOutTile - 1) / kOutTile;
@@ -916,6 +923,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       const uint64_t r0 = a.row_start[qlo + threadIdx.x];
       s_qr0[threadIdx.x] = r0;
       s_qp0[threadIdx.x] = a.pfx[r0 < R ? r0 : R];
+      s_ck[threadIdx.x] = 0;
     }
     __syncthreads();
     if (staged && qhi - qlo == 1 && a.desc[qlo].out_off == kNoSpace) {
@@ -986,7 +994,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
             if (s_qr0[m] <= g) ql = m;
             else qh = m;
           }
-          emit_row(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, s_pfx[l], s_eoff[l]);
+          // checksum shares gather in LDS, one global add per query and tile
+          const uint64_t rh = emit_row(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, s_pfx[l], s_eoff[l]);
+          if (rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
         } else if (staged) {
           uint32_t l = 0, h = uint32_t(nr);
           while (h - l > 1) {
@@ -995,7 +1005,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
             else h = m;
           }
           const uint64_t g = lo + l;
-          emit_row(a, query_in(a.row_start, qlo, qhi, g), g, j, s_pfx[l], s_eoff[l]);
+          const uint32_t q = query_in(a.row_start, qlo, qhi, g);
+          add_checksum(a, q, emit_row(a, q, g, j, s_pfx[l], s_eoff[l]));
         } else {
           uint64_t l = lo, h = hi;
           while (h - l > 1) {
@@ -1003,8 +1014,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
             if (a.pfx[m] <= j) l = m;
             else h = m;
           }
-          emit_row(a, query_in(a.row_start, qlo, qhi, l), l, j, a.pfx[l], a.eoff[l]);
+          const uint32_t q = query_in(a.row_start, qlo, qhi, l);
+          add_checksum(a, q, emit_row(a, q, l, j, a.pfx[l], a.eoff[l]));
         }
+      }
+      if (a.ck && qhi - qlo > 1 && qstaged) {
+        __syncthreads();
+        if (threadIdx.x < qhi - qlo && s_ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], s_ck[threadIdx.x]);
       }
     }
     __syncthreads();
@@ -1023,7 +1039,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
     if (!c) continue;
     const uint32_t q = find_query(a.row_start, a.nq, g);
     const uint64_t p = a.pfx[g];
-    for (uint32_t k = gl; k < c; k += LPR) emit_row(a, q, g, p + k);
+    uint64_t hsum = 0;
+    for (uint32_t k = gl; k < c; k += LPR) hsum += emit_row(a, q, g, p + k);
+    if (a.ck) add_checksum(a, q, hsum);
   }
 }
 

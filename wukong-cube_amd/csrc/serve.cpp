@@ -89,6 +89,7 @@ struct Server {
     if (rc) return rc;
     std::vector<int> segs;
     uint64_t R = 0;
+    uint32_t nfinal = 0;
     for (uint32_t i = 0; i < n; i++) {
       const uint32_t q = F[a + i];
       const Plan& p = plans[q];
@@ -97,6 +98,7 @@ struct Server {
       d.in_rows = qs[q].k == 0 ? 1 : qs[q].rows;
       d.in_off = qs[q].k == 0 ? 0 : qs[q].off;
       dev->h_desc[i] = d;
+      nfinal += d.final_step;
       dev->h_rs[i] = R;
       R += d.in_rows;
       if (d.seg >= 0 && std::find(segs.begin(), segs.end(), d.seg) == segs.end()) segs.push_back(d.seg);
@@ -125,7 +127,8 @@ struct Server {
       }
       StepArgs args = make_args(dev, 0, n, tin, tout, dev->tcap, false);
       // the gather sums the row hashes of each final table as it writes it
-      if (checksum_now) {
+      const bool ck = checksum_now && nfinal > 0;
+      if (ck) {
         launch_fill_u64(d_sums, n, 0, dev->stream);
         args.ck = reinterpret_cast<unsigned long long*>(d_sums);
       }
@@ -134,8 +137,7 @@ struct Server {
       launch_probe(args, dev->opts.probe_width, R, dev->stream);
       launch_expand_tail(args, dev->opts.rows_per_wave, dev->stream);
       timed_end(dev, ev, "serve:step_kernels", 0);
-      if (checksum_now &&
-          hip_failed(hipMemcpyAsync(h_sums, d_sums, n * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
+      if (ck && hip_failed(hipMemcpyAsync(h_sums, d_sums, n * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
       }
@@ -159,7 +161,7 @@ struct Server {
           ua = std::max<uint64_t>(ua, dev->h_desc[i].out_off + dev->h_desc[i].out_rows * dev->h_desc[i].out_cols);
         }
         *used_a = ua;
-        if (checksum_now)
+        if (ck)
           for (uint32_t i = 0; i < n; i++)
             if (dev->h_desc[i].final_step) res[a + i].in_off = h_sums[i];  // carry the checksum
         return WK_OK;
