@@ -180,6 +180,10 @@ int wk_dev_stats_get(wk_dev* dev, wk_dev_stats* stats);
 int wk_dev_kernel_stats(wk_dev* dev, wk_kernel_stat* out, uint32_t cap, uint32_t* n);
 int wk_dev_reset_kernel_stats(wk_dev* dev);
 int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave);
+/* Turn per-kernel and host-phase timing (opts.profile) on or off; timings
+ * already taken are kept.  No counterpart in the reference: it lets one
+ * handle time single queries and then serve without event overhead. */
+int wk_dev_set_profile(wk_dev* dev, int32_t on);
 int wk_dev_sync(wk_dev* dev);
 /* Load every segment that fits the budget (warm cache). */
 int wk_dev_load_all(wk_dev* dev);

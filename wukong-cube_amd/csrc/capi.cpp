@@ -299,6 +299,13 @@ int wk_dev_reset_kernel_stats(wk_dev* dev) {
   return rc;
 }
 
+int wk_dev_set_profile(wk_dev* dev, int32_t on) {
+  if (!dev) return WK_E_INVALID_ARG;
+  int rc = flush_timings(dev);
+  dev->opts.profile = on ? 1 : 0;
+  return rc;
+}
+
 int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
   if (!dev) return WK_E_INVALID_ARG;
   if (probe_width != 1 && probe_width != 2 && probe_width != 4 && probe_width != 8) return WK_E_INVALID_ARG;

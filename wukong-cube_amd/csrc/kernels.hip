@@ -833,8 +833,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
 // Writes output row j of query q: input row g (whose first output is row
 // pg of the step) and, for an expansion, edge eo + (j - pg) of its list.
 // r0 is the query's first input row and p0 = pfx[r0] its first output row.
-// Returns the row's hash when the step keeps checksums of final tables
-// (a.ck) and this is one, else 0; the caller adds it to the query's sum.
+// With CK, returns the row's hash when the step keeps checksums of final
+// tables (a.ck) and this is one, else 0; the caller adds it to the query's
+// sum.  Without CK no hashing code is generated at all.
+template <bool CK>
 __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t r0, uint64_t p0,
                                              uint64_t g, uint64_t j, uint64_t pg, uint32_t eo) {
   const StepDesc& d = a.desc[q];
@@ -843,7 +845,7 @@ __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint
   uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
   const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
   const uint32_t ic = d.in_cols;
-  const bool ck = a.ck && d.final_step;
+  const bool ck = CK && a.ck && d.final_step;
   uint64_t h = row_hash_init(d.out_cols);
   for (uint32_t c = 0; c < ic; ++c) {
     const uint32_t x = src[c];
@@ -859,14 +861,16 @@ __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint
   return ck ? h : 0;
 }
 
+template <bool CK>
 __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
                                              uint32_t eo) {
   const uint64_t r0 = a.row_start[q];
-  return emit_row(a, q, r0, a.pfx[r0], g, j, pg, eo);
+  return emit_row<CK>(a, q, r0, a.pfx[r0], g, j, pg, eo);
 }
 
+template <bool CK>
 __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
-  return emit_row(a, q, g, j, a.pfx[g], a.eoff[g]);
+  return emit_row<CK>(a, q, g, j, a.pfx[g], a.eoff[g]);
 }
 
 __device__ __forceinline__ void add_checksum(const StepArgs& a, uint32_t q, uint64_t h) {
@@ -882,6 +886,7 @@ __device__ __forceinline__ void add_checksum(const StepArgs& a, uint32_t q, uint
 constexpr int kStage = kOutTile + 1;
 constexpr int kQStage = kBlock;  // queries of a mixed tile staged in LDS
 
+template <bool CK>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   if (a.st->status) return;
   __shared__ uint64_t s_pfx[kStage];
@@ -940,7 +945,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       uint32_t* const dbase = (d.final_step ? a.arena : a.tout) + d.out_off;
       const uint32_t* const sbase = a.tin + d.in_off;
       const DevSeg sg = a.segs[expand ? d.seg : 0];
-      const bool ck = a.ck && d.final_step;
+      const bool ck = CK && a.ck && d.final_step;
       uint64_t hsum = 0;  // row hashes of this thread's final rows
       uint32_t l = 0;  // source rows only move forward as j grows
 #pragma unroll
@@ -995,8 +1000,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
             else qh = m;
           }
           // checksum shares gather in LDS, one global add per query and tile
-          const uint64_t rh = emit_row(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, s_pfx[l], s_eoff[l]);
-          if (rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
+          const uint64_t rh = emit_row<CK>(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, s_pfx[l], s_eoff[l]);
+          if (CK && rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
         } else if (staged) {
           uint32_t l = 0, h = uint32_t(nr);
           while (h - l > 1) {
@@ -1006,7 +1011,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
           }
           const uint64_t g = lo + l;
           const uint32_t q = query_in(a.row_start, qlo, qhi, g);
-          add_checksum(a, q, emit_row(a, q, g, j, s_pfx[l], s_eoff[l]));
+          const uint64_t rh = emit_row<CK>(a, q, g, j, s_pfx[l], s_eoff[l]);
+          if (CK) add_checksum(a, q, rh);
         } else {
           uint64_t l = lo, h = hi;
           while (h - l > 1) {
@@ -1015,10 +1021,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
             else h = m;
           }
           const uint32_t q = query_in(a.row_start, qlo, qhi, l);
-          add_checksum(a, q, emit_row(a, q, l, j, a.pfx[l], a.eoff[l]));
+          const uint64_t rh = emit_row<CK>(a, q, l, j, a.pfx[l], a.eoff[l]);
+          if (CK) add_checksum(a, q, rh);
         }
       }
-      if (a.ck && qhi - qlo > 1 && qstaged) {
+      if (CK && a.ck && qhi - qlo > 1 && qstaged) {
         __syncthreads();
         if (threadIdx.x < qhi - qlo && s_ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], s_ck[threadIdx.x]);
       }
@@ -1040,7 +1047,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
     const uint32_t q = find_query(a.row_start, a.nq, g);
     const uint64_t p = a.pfx[g];
     uint64_t hsum = 0;
-    for (uint32_t k = gl; k < c; k += LPR) hsum += emit_row(a, q, g, p + k);
+    for (uint32_t k = gl; k < c; k += LPR) hsum += emit_row<true>(a, q, g, p + k);
     if (a.ck) add_checksum(a, q, hsum);
   }
 }
@@ -1157,7 +1164,11 @@ void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint,
   if (rows_per_wave <= 0) {
     const unsigned grid =
         out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : 4 * kPersistentGrid;
-    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(kBlock), 0, s, a);
+    // the checksum variant only when final tables are checksummed (serving)
+    if (a.ck)
+      hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(kBlock), 0, s, a);
     return;
   }
   // rows per wavefront -> lanes per row

@@ -132,11 +132,8 @@ struct Server {
         launch_fill_u64(d_sums, n, 0, dev->stream);
         args.ck = reinterpret_cast<unsigned long long*>(d_sums);
       }
-      hipEvent_t ev = nullptr;
-      timed_begin(dev, &ev);
       launch_probe(args, dev->opts.probe_width, R, dev->stream);
       launch_expand_tail(args, dev->opts.rows_per_wave, dev->stream);
-      timed_end(dev, ev, "serve:step_kernels", 0);
       if (ck && hip_failed(hipMemcpyAsync(h_sums, d_sums, n * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
@@ -319,7 +316,6 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     rc = srv.step_range(inflight, 0, inflight.size(), cur, 0, 0, &used_t, &used_a, res, failed, true);
     if (rc) break;
     host_phase(dev, "serve:iteration", t_it);
-    if (dev->opts.profile && (rc = flush_timings(dev)) != WK_OK) break;
     const double t = now_s();
     std::vector<uint32_t> still;
     for (size_t i = 0; i < inflight.size(); i++) {

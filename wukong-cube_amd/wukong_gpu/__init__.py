@@ -124,6 +124,7 @@ SIGNATURES = {
     "wk_dev_kernel_stats": (C.c_int, [P, C.POINTER(KernelStat), C.c_uint32, U32P]),
     "wk_dev_reset_kernel_stats": (C.c_int, [P]),
     "wk_dev_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32]),
+    "wk_dev_set_profile": (C.c_int, [P, C.c_int32]),
     "wk_dev_sync": (C.c_int, [P]),
     "wk_dev_load_all": (C.c_int, [P]),
     "wk_seg_ensure": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
@@ -393,6 +394,9 @@ class Device:
 
     def set_tuning(self, probe_width, rows_per_wave):
         _check(lib().wk_dev_set_tuning(self._h, probe_width, rows_per_wave))
+
+    def set_profile(self, on):
+        _check(lib().wk_dev_set_profile(self._h, 1 if on else 0))
 
     def stats(self):
         s = DevStats()
