@@ -311,3 +311,19 @@ def test_batch_drops_only_the_final_tables_that_do_not_fit(univ10_store, univ10_
         else:
             assert rc == wg.WK_E_NOMEM_BUDGET, (i, rc)
     dev.close()
+
+
+def test_profile_switch(univ10_store, univ10_oracle):
+    """Per-kernel timing follows wk_dev_set_profile; results do not."""
+    dev = wg.Device(univ10_store, table_budget=64 << 20, profile=1)
+    pats, nv = Q.chain3()
+    check_query(dev, univ10_oracle, pats, nv)
+    assert any(k.startswith("k_probe") for k in dev.kernel_stats())
+    dev.set_profile(False)
+    dev.reset_kernel_stats()
+    check_query(dev, univ10_oracle, pats, nv)
+    assert dev.kernel_stats() == {}
+    dev.set_profile(True)
+    check_query(dev, univ10_oracle, pats, nv)
+    assert dev.kernel_stats()["k_probe"]["launches"] == 3
+    dev.close()
