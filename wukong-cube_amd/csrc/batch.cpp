@@ -66,7 +66,7 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   for (uint32_t i = 0; i < nq; i++) {
     const Plan& p = plans[q0 + i];
     const uint32_t ns = p.status == WK_OK ? uint32_t(p.steps.size()) : 0;
-    for (uint32_t k = 0; k <= S; k++) {
+    for (uint32_t k = 0; k < S; k++) {
       StepDesc* d = dev->h_desc + k * qc + i;
       if (k < ns) {
         *d = p.steps[k];
@@ -79,10 +79,12 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
       }
     }
   }
+  // step 0: one input row (the constant start) per query that has steps
   uint64_t nvalid = 0;
   for (uint32_t i = 0; i < nq; i++) {
     dev->h_rs[i] = nvalid;
-    nvalid += dev->h_desc[i].in_rows;
+    const Plan& p = plans[q0 + i];
+    nvalid += (p.status == WK_OK && !p.steps.empty()) ? 1 : 0;
   }
   dev->h_rs[nq] = nvalid;
   t = host_phase(dev, "host:batch_fill", t);
@@ -92,7 +94,8 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     unpin_segments(dev, segs);
     return code;
   };
-  if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, (S + 1) * qc * sizeof(StepDesc), hipMemcpyHostToDevice,
+  // steps 0..S-1 are read; the last step writes no next-step descriptors
+  if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, ((S - 1) * qc + nq) * sizeof(StepDesc), hipMemcpyHostToDevice,
                      dev->stream)) ||
       hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
       hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState), hipMemcpyHostToDevice,

@@ -54,24 +54,8 @@ struct Server {
   wk_dev* dev;
   const std::vector<Plan>& plans;
   std::vector<QState>& qs;
-  uint64_t* d_sums = nullptr;
-  uint64_t* h_sums = nullptr;
-  uint32_t cap = 0;
 
   Server(wk_dev* d, const std::vector<Plan>& p, std::vector<QState>& s) : dev(d), plans(p), qs(s) {}
-  ~Server() {
-    if (d_sums) (void)hipFree(d_sums);
-    if (h_sums) (void)hipHostFree(h_sums);
-  }
-  int reserve(uint32_t n) {
-    if (n <= cap) return WK_OK;
-    if (d_sums) (void)hipFree(d_sums);
-    if (h_sums) (void)hipHostFree(h_sums);
-    cap = std::max<uint32_t>(n, 1024);
-    WK_HIP(hipMalloc(&d_sums, cap * sizeof(uint64_t)));
-    WK_HIP(hipHostMalloc(&h_sums, cap * sizeof(uint64_t), hipHostMallocDefault));
-    return WK_OK;
-  }
 
   // One step for queries F[a, b): outputs from tout_base / arena_base.  On
   // overflow the range is halved; a single query that cannot fit fails.
@@ -83,10 +67,22 @@ struct Server {
     *used_t = tout_base;
     *used_a = arena_base;
     if (n == 0) return WK_OK;
-    int rc = reserve_steps(dev, 1, n);
+    // two descriptor sets of at least 2 queries hold the staging block below
+    int rc = reserve_steps(dev, 1, std::max<uint32_t>(n, 2));
     if (rc) return rc;
-    rc = reserve(n);
-    if (rc) return rc;
+    // one staging block each way: [descriptors n][row starts n+1][state][sums n]
+    // in the step-descriptor buffers (sized for two steps of nq_cap queries),
+    // so an iteration is one upload and one download
+    const size_t off_rs = size_t(n) * sizeof(StepDesc);
+    const size_t off_st = (off_rs + size_t(n + 1) * sizeof(uint64_t) + 15) & ~size_t(15);
+    const size_t off_ck = off_st + sizeof(StepState);
+    const size_t end_ck = off_ck + size_t(n) * sizeof(uint64_t);
+    uint8_t* const hb = reinterpret_cast<uint8_t*>(dev->h_desc);
+    uint8_t* const db = reinterpret_cast<uint8_t*>(dev->d_desc);
+    StepDesc* const hdesc = dev->h_desc;
+    uint64_t* const hrs = reinterpret_cast<uint64_t*>(hb + off_rs);
+    StepState* const hst = reinterpret_cast<StepState*>(hb + off_st);
+    const uint64_t* const hsum = reinterpret_cast<const uint64_t*>(hb + off_ck);
     std::vector<int> segs;
     uint64_t R = 0;
     uint32_t nfinal = 0;
@@ -97,13 +93,13 @@ struct Server {
       d.final_step = (qs[q].k + 1 == p.steps.size()) ? 1 : 0;
       d.in_rows = qs[q].k == 0 ? 1 : qs[q].rows;
       d.in_off = qs[q].k == 0 ? 0 : qs[q].off;
-      dev->h_desc[i] = d;
+      hdesc[i] = d;
       nfinal += d.final_step;
-      dev->h_rs[i] = R;
+      hrs[i] = R;
       R += d.in_rows;
       if (d.seg >= 0 && std::find(segs.begin(), segs.end(), d.seg) == segs.end()) segs.push_back(d.seg);
     }
-    dev->h_rs[n] = R;
+    hrs[n] = R;
     if (R > dev->rcap) {
       if (n == 1) {
         failed.push_back(int(a));
@@ -112,55 +108,48 @@ struct Server {
       goto split;
     }
     {
-      dev->h_state[0] = StepState{};
-      dev->h_state[0].tout_base = tout_base;
-      dev->h_state[0].arena_base = arena_base;
+      *hst = StepState{};
+      hst->tout_base = tout_base;
+      hst->arena_base = arena_base;
       rc = ensure_segments(dev, segs);
       if (rc) return rc;
       const uint32_t* tin = dev->tbuf[cur];
       uint32_t* tout = dev->tbuf[cur ^ 1];
-      if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, n * sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream)) ||
-          hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
-          hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream))) {
+      if (hip_failed(hipMemcpyAsync(db, hb, off_ck, hipMemcpyHostToDevice, dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
       }
       StepArgs args = make_args(dev, 0, n, tin, tout, dev->tcap, false);
+      args.row_start = reinterpret_cast<uint64_t*>(db + off_rs);
+      args.st = reinterpret_cast<StepState*>(db + off_st);
       // the gather sums the row hashes of each final table as it writes it
+      // (the query scan zeroes the sums first)
       const bool ck = checksum_now && nfinal > 0;
-      if (ck) {
-        launch_fill_u64(d_sums, n, 0, dev->stream);
-        args.ck = reinterpret_cast<unsigned long long*>(d_sums);
-      }
+      if (ck) args.ck = reinterpret_cast<unsigned long long*>(db + off_ck);
       launch_probe(args, dev->opts.probe_width, R, dev->stream);
       launch_expand_tail(args, dev->opts.rows_per_wave, dev->stream);
-      if (ck && hip_failed(hipMemcpyAsync(h_sums, d_sums, n * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream))) {
-        unpin_segments(dev, segs);
-        return WK_E_HIP;
-      }
-      if (hip_failed(hipMemcpyAsync(dev->h_desc, dev->d_desc, n * sizeof(StepDesc), hipMemcpyDeviceToHost, dev->stream)) ||
-          hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
+      if (hip_failed(hipMemcpyAsync(hb, db, ck ? end_ck : off_ck, hipMemcpyDeviceToHost, dev->stream)) ||
           hip_failed(hipStreamSynchronize(dev->stream))) {
         unpin_segments(dev, segs);
         return WK_E_HIP;
       }
       unpin_segments(dev, segs);
-      if (dev->h_state[0].status == 0) {
-        for (uint32_t i = 0; i < n; i++) res[a + i] = dev->h_desc[i];
-        *used_t = tout_base + dev->h_state[0].out_elems;
+      if (hst->status == 0) {
+        for (uint32_t i = 0; i < n; i++) res[a + i] = hdesc[i];
+        *used_t = tout_base + hst->out_elems;
         uint64_t ua = arena_base;
         for (uint32_t i = 0; i < n; i++) {
-          if (!dev->h_desc[i].final_step) continue;
-          if (dev->h_desc[i].out_off == kNoSpace) {
+          if (!hdesc[i].final_step) continue;
+          if (hdesc[i].out_off == kNoSpace) {
             failed.push_back(int(a + i));  // its final table did not fit the arena
             continue;
           }
-          ua = std::max<uint64_t>(ua, dev->h_desc[i].out_off + dev->h_desc[i].out_rows * dev->h_desc[i].out_cols);
+          ua = std::max<uint64_t>(ua, hdesc[i].out_off + hdesc[i].out_rows * hdesc[i].out_cols);
         }
         *used_a = ua;
         if (ck)
           for (uint32_t i = 0; i < n; i++)
-            if (dev->h_desc[i].final_step) res[a + i].in_off = h_sums[i];  // carry the checksum
+            if (hdesc[i].final_step) res[a + i].in_off = hsum[i];  // carry the checksum
         return WK_OK;
       }
       if (n == 1) {
