@@ -139,6 +139,39 @@ def run_single(dev, singles, args):
     return single
 
 
+def run_partition(dev, single, args, owners=(8, 64)):
+    """Fork-join split of the largest single-query result among 8 and 64
+    owners (row[0] % n, input order kept within each owner).  Minimal bytes
+    are one read and one write of the table; the timed call includes the
+    output allocation and the owners' heads coming back to the host."""
+    if not single:
+        return None
+    name = max(single, key=lambda k: single[k]["rows"])
+    host = single[name]["_table"]
+    tbl = dev.upload(host)
+    res = {"table": name, "rows": int(host.shape[0]), "cols": int(host.shape[1]), "alg_bytes": 2.0 * host.size * 4,
+           "owners": {}}
+    for n in owners:
+        for _ in range(args.warmup):
+            out, _, _ = dev.partition(tbl, 0, n)
+            out.free()
+        dev.sync()
+        lat = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            out, sizes, heads = dev.partition(tbl, 0, n)
+            lat.append((time.perf_counter() - t) * 1e3)
+            out.free()
+        assert int(sizes.sum()) == host.shape[0]
+        ms = float(np.median(lat))
+        gbps = res["alg_bytes"] / (ms * 1e-3) / 1e9
+        res["owners"][str(n)] = {"ms_p50": ms, "GBps": gbps, "max_owner_rows": int(sizes.max())}
+        log("split of %s (%d x %d) among %d owners: %.3f ms, %.0f GB/s" % (name, host.shape[0], host.shape[1], n,
+                                                                          ms, gbps))
+    tbl.free()
+    return res
+
+
 def roofline_of(single, args):
     """The dominant kernel over the single queries: algorithmic bytes per
     launch over its average launch time."""
@@ -190,13 +223,14 @@ def main():
     # ---------------------------------------------------------------- single
     single = {} if args.batch_only else run_single(dev, singles, args)
     roofline = roofline_of(single, args)
+    partition = run_partition(dev, single, args)
     edges = sum(r["rows"] for r in single.values())
     if args.single_only:
         for r in single.values():
             for k in ("_table", "_pats", "_nv"):
                 r.pop(k, None)
         print(json.dumps({"metric": "single-query kernels", "single_query": single, "roofline": roofline,
-                          "config": cfg}))
+                          "partition": partition, "config": cfg}))
         dev.close()
         return
 
@@ -353,7 +387,7 @@ def main():
         "per_query_qps": qps_perq, "batched_vs_per_query": qps_batched / qps_perq,
         "single_query": single,
         "edges_per_s": (edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values())) if single else None,
-        "serve": serve, "roofline": roofline, "cpu_baseline": cpu, "parity_gate": gate,
+        "serve": serve, "partition": partition, "roofline": roofline, "cpu_baseline": cpu, "parity_gate": gate,
         "gstore": {"h2d_bytes": dev.stats()["h2d_bytes"], "segments_evicted": dev.stats()["segments_evicted"]},
     })
     dev.close()

@@ -197,6 +197,19 @@ int wk_table_upload(wk_dev* dev, const uint32_t* host, uint64_t rows, uint32_t c
 int wk_table_shape(const wk_table* t, uint64_t* rows, uint32_t* cols);
 int wk_table_download(wk_dev* dev, const wk_table* t, uint32_t* dst, uint64_t cap_elems);
 int wk_table_free(wk_dev* dev, wk_table* t);
+/* Fork-join dispatch of a history table among nparts owners (1..64): out
+ * holds in's rows grouped by owner row[col] % nparts, in row order within
+ * each owner, owner p at rows [heads[p], heads[p] + sizes[p]).  Replaces
+ * SPARQLEngine::generate_sub_query's split (core/engine/sparql.hpp:741-793,
+ * PARTITION = vid % num_servers, core/store/vertex.hpp:59) and the GPU
+ * engine's gpu_shuffle_result_buf / gpu_split_result_buf
+ * (src/gpu/gpu_hash.cu:601-762; GPUEngine::generate_sub_query,
+ * src/gpu/gpu_engine.hpp:340-393), whose atomic positions lose the order.
+ * WK_E_INVALID_ARG: nparts outside 1..64, or col >= cols of a non-empty
+ * table.  WK_E_NOMEM_BUDGET: the owner x tile histogram does not fit the
+ * device's row scratch, or the output table cannot be allocated. */
+int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t nparts, wk_table** out,
+                       uint64_t* sizes, uint64_t* heads);
 
 /* --------------------------------------------------------------- patterns */
 

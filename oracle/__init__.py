@@ -144,6 +144,19 @@ def final_process(table, v2c, required, distinct=False, orders=(), offset=0, lim
     return out
 
 
+def partition_rows(table, col, nparts):
+    """Split of a history table among nparts owners, as
+    SPARQLEngine::generate_sub_query does it (core/engine/sparql.hpp:741-793):
+    each row, in order, is appended to owner PARTITION(row[col]) =
+    row[col] % nparts (core/store/vertex.hpp:59, utils/math.hpp:52-56).
+    Returns the owners' tables concatenated in owner order, with the sizes."""
+    t = np.ascontiguousarray(table, dtype=np.uint32)
+    owner = t[:, col].astype(np.uint64) % nparts if t.shape[0] else np.zeros(0, np.uint64)
+    order = np.argsort(owner, kind="stable")
+    sizes = np.bincount(owner.astype(np.int64), minlength=nparts).astype(np.uint64)
+    return t[order], sizes
+
+
 def sorted_rows(t):
     """Canonical form for multiset comparison: rows sorted lexicographically."""
     t = np.asarray(t, dtype=np.uint32)

@@ -327,3 +327,35 @@ def test_profile_switch(univ10_store, univ10_oracle):
     check_query(dev, univ10_oracle, pats, nv)
     assert dev.kernel_stats()["k_probe"]["launches"] == 3
     dev.close()
+
+
+@pytest.mark.parametrize("rows,cols,nparts", [(0, 2, 3), (1, 1, 1), (1000, 3, 2), (1025, 2, 8), (5000, 4, 64),
+                                              (70000, 3, 5), (3000, 7, 16), (2100, 11, 3), (40000, 13, 7),
+                                              (200000, 2, 64)])
+def test_partition_matches_oracle(univ10_dev, rows, cols, nparts):
+    """Fork-join dispatch on the device equals the CPU engine's split row for
+    row (same owners, same order within each owner), on ragged sizes around
+    the 1,024-row tile, tables narrow enough to sort in LDS and wider ones
+    (11 and 13 columns), and a histogram of several scan chunks (64 owners x 196
+    tiles)."""
+    rng = np.random.default_rng(rows + 31 * nparts)
+    t = rng.integers(0, 1 << 24, size=(rows, cols), dtype=np.uint32)
+    col = cols - 1
+    want, wsizes = oracle.partition_rows(t, col, nparts)
+    dt = univ10_dev.upload(t)
+    out, sizes, heads = univ10_dev.partition(dt, col, nparts)
+    assert out.shape == (rows, cols)
+    assert np.array_equal(sizes, wsizes)
+    assert np.array_equal(heads, np.concatenate([[0], np.cumsum(wsizes)[:-1]]).astype(np.uint64))
+    if rows:
+        assert np.array_equal(out.numpy(), want)
+    out.free()
+    dt.free()
+
+
+def test_partition_rejects_bad_arguments(univ10_dev):
+    dt = univ10_dev.upload(np.arange(12, dtype=np.uint32).reshape(4, 3))
+    for col, n in [(3, 2), (0, 0), (0, 65)]:
+        with pytest.raises(wg.WkError):
+            univ10_dev.partition(dt, col, n)
+    dt.free()

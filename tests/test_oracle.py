@@ -249,3 +249,41 @@ def test_final_process_projection_distinct_order_limit():
     # empty table stays untouched
     e = np.zeros((0, 3), np.uint32)
     assert oracle.final_process(e, v2c, [-1]).shape == (0, 3)
+
+
+def test_partition_rows_matches_the_reference_loop():
+    """oracle.partition_rows against a literal restatement of
+    generate_sub_query's loop (append row i to owner row[col] % n, in order),
+    including empty tables, one owner and owners with no rows."""
+    rng = np.random.default_rng(5)
+    for rows, cols, n in [(0, 2, 3), (1, 1, 1), (7, 3, 4), (1000, 2, 7), (3000, 4, 64)]:
+        t = rng.integers(0, 1 << 20, size=(rows, cols), dtype=np.uint32)
+        col = int(rng.integers(0, cols))
+        got, sizes = oracle.partition_rows(t, col, n)
+        subs = [[] for _ in range(n)]
+        for r in range(rows):
+            subs[int(t[r, col]) % n].append(t[r])
+        want = np.array([row for s in subs for row in s], dtype=np.uint32).reshape(rows, cols)
+        assert np.array_equal(got, want)
+        assert [len(s) for s in subs] == sizes.tolist()
+
+
+def test_partition_rows_invariants():
+    """What any split of rows among owners must satisfy, on rows tagged with
+    their input index: a permutation of the input, owner blocks in owner
+    order with the owner's sizes, every row in the block of row[col] % n, and
+    input order kept inside each block."""
+    rng = np.random.default_rng(9)
+    for rows, n in [(1, 1), (50, 3), (4096, 64), (10000, 5)]:
+        t = np.stack([np.arange(rows, dtype=np.uint32),
+                      rng.integers(0, 1 << 31, size=rows, dtype=np.uint32)], axis=1)
+        got, sizes = oracle.partition_rows(t, 1, n)
+        assert sorted(got[:, 0].tolist()) == list(range(rows))
+        assert int(sizes.sum()) == rows
+        start = 0
+        for p in range(n):
+            blk = got[start:start + int(sizes[p])]
+            assert np.all(blk[:, 1] % n == p)
+            assert np.all(np.diff(blk[:, 0].astype(np.int64)) > 0)
+            assert np.array_equal(blk, t[blk[:, 0]])
+            start += int(sizes[p])

@@ -407,6 +407,42 @@ int wk_table_free(wk_dev* dev, wk_table* t) {
   return WK_OK;
 }
 
+int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t nparts, wk_table** out,
+                       uint64_t* sizes, uint64_t* heads) {
+  if (!dev || !in || !out || !sizes || !heads) return WK_E_INVALID_ARG;
+  *out = nullptr;
+  if (nparts == 0 || nparts > kMaxParts || (in->rows && col >= in->cols)) return WK_E_INVALID_ARG;
+  const uint64_t rows = in->rows;
+  // scratch: the step prefix array (rcap + 1 words) holds the histogram,
+  // the chunk totals and the heads
+  if (partition_scratch_words(rows, nparts) > dev->rcap + 1) return WK_E_NOMEM_BUDGET;
+  wk_table* t = new (std::nothrow) wk_table();
+  if (!t) return WK_E_NOMEM_HOST;
+  t->rows = rows;
+  t->cols = in->cols;
+  for (uint32_t p = 0; p < nparts; p++) sizes[p] = heads[p] = 0;
+  if (rows * in->cols) {
+    if (hipMalloc(&t->d, rows * in->cols * sizeof(uint32_t)) != hipSuccess) {
+      delete t;
+      return WK_E_NOMEM_BUDGET;
+    }
+    const uint64_t* dh = launch_partition(in->d, rows, in->cols, col, nparts, dev->pfx, t->d, dev->stream);
+    std::vector<uint64_t> h(size_t(nparts) + 1);
+    if (hip_failed(hipMemcpyAsync(h.data(), dh, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream)) ||
+        hip_failed(hipStreamSynchronize(dev->stream))) {
+      (void)hipFree(t->d);
+      delete t;
+      return WK_E_HIP;
+    }
+    for (uint32_t p = 0; p < nparts; p++) {
+      heads[p] = h[p];
+      sizes[p] = h[p + 1] - h[p];
+    }
+  }
+  *out = t;
+  return WK_OK;
+}
+
 // --------------------------------------------------------------- patterns
 
 static int pattern_step(wk_dev* dev, StepDesc sd, const wk_table* in, wk_table** out) {

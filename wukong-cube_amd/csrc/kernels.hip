@@ -1054,6 +1054,204 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
   }
 }
 
+// ---- fork-join dispatch: rows grouped by owner = row[col] % nparts --------
+//
+// The split of a history table among its next-hop owners before a remote
+// step (CPU: SPARQLEngine::generate_sub_query, sparql.hpp:741-793, rows
+// appended in order; GPU: gpu_shuffle_result_buf / gpu_split_result_buf,
+// gpu_hash.cu:601-762, positions by atomics).  This keeps the CPU order: a
+// row's place is its owner's base, plus the owner's rows in earlier tiles,
+// plus its rank among the owner's rows earlier in its own tile.
+
+__device__ __forceinline__ uint32_t row_owner(const uint32_t* in, uint32_t cols, uint32_t col, uint32_t nparts,
+                                              uint64_t r) {
+  return in[r * cols + col] % nparts;
+}
+
+// Per-tile owner counts, owner-major: hist[p * ntiles + t].
+__global__ __launch_bounds__(kBlock) void k_part_hist(const uint32_t* in, uint64_t rows, uint32_t cols,
+                                                      uint32_t col, uint32_t nparts, uint64_t* hist) {
+  __shared__ uint32_t s_h[kMaxParts];
+  const uint64_t ntiles = (rows + kTile - 1) / kTile;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) s_h[p] = 0;
+    __syncthreads();
+    for (uint32_t k = 0; k < kTile / kBlock; ++k) {
+      const uint64_t r = t * kTile + uint64_t(k) * kBlock + threadIdx.x;
+      if (r < rows) atomicAdd(&s_h[row_owner(in, cols, col, nparts, r)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) hist[uint64_t(p) * ntiles + t] = s_h[p];
+    __syncthreads();
+  }
+}
+
+// Local exclusive scan of the histogram in chunks of kPartChunk entries
+// (8 consecutive entries per thread); chunk totals go to csum[chunk].
+constexpr uint32_t kPartChunk = kBlock * 8;
+
+__global__ __launch_bounds__(kBlock) void k_part_scan_local(uint64_t* v, uint64_t n, uint64_t* csum) {
+  const uint64_t nch = (n + kPartChunk - 1) / kPartChunk;
+  for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    const uint64_t lo = c * kPartChunk + uint64_t(threadIdx.x) * 8;
+    uint64_t x[8], s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      x[k] = lo + k < n ? v[lo + k] : 0;
+      s += x[k];
+    }
+    uint64_t total = 0;
+    uint64_t ex = block_excl_scan<kBlock>(s, &total);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (lo + k < n) v[lo + k] = ex;
+      ex += x[k];
+    }
+    if (threadIdx.x == 0) csum[c] = total;
+  }
+}
+
+// Exclusive scan of the chunk totals by one 1,024-thread block, then each
+// owner's first output row: heads[p] = global position of (owner p, tile 0);
+// heads[nparts] = the row count.
+__global__ __launch_bounds__(1024) void k_part_scan_chunks(const uint64_t* hist, uint64_t* csum, uint64_t nch,
+                                                           uint64_t ntiles, uint32_t nparts, uint64_t* heads) {
+  const uint64_t per = (nch + 1023) / 1024;
+  uint64_t lo = uint64_t(threadIdx.x) * per;
+  if (lo > nch) lo = nch;
+  const uint64_t hi = (lo + per < nch) ? lo + per : nch;
+  uint64_t s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += csum[i];
+  uint64_t total = 0;
+  uint64_t ex = block_excl_scan<1024>(s, &total);
+  for (uint64_t i = lo; i < hi; ++i) {
+    const uint64_t x = csum[i];
+    csum[i] = ex;
+    ex += x;
+  }
+  __syncthreads();  // the block's csum writes are visible to the block
+  if (threadIdx.x < nparts) {
+    const uint64_t i = uint64_t(threadIdx.x) * ntiles;
+    heads[threadIdx.x] = hist[i] + csum[i / kPartChunk];
+  }
+  if (threadIdx.x == 0) heads[nparts] = total;
+}
+
+// Scatter: wave w of a block takes rows [w * 256, w * 256 + 256) of the
+// tile, 64 per pass.  A row's rank among the wave's rows of its owner in one
+// pass comes from one ballot per owner bit (the lanes whose owner equals
+// this lane's); wave-private counters in LDS carry it across passes, and one
+// barrier per tile adds the rows of the owner in earlier waves.  So a row's
+// place is the owner's base, plus the owner's rows in earlier tiles, plus
+// its rank among the tile's rows of its owner: the input order.
+//
+// MAXC > 0 (tables of at most MAXC columns): a lane loads all its rows of
+// the tile into registers at once, takes the owners from them, and writes
+// each row to its place.  MAXC = 0 (wider tables): a lane reads only the
+// owner words; the rows' places go to LDS, and the block then copies the
+// tile word by word, consecutive lanes on consecutive words.
+template <int MAXC>
+__global__ __launch_bounds__(kBlock) void k_part_scatter(const uint32_t* __restrict__ in, uint64_t rows,
+                                                         uint32_t cols, uint32_t col, uint32_t nparts,
+                                                         const uint64_t* __restrict__ hist,
+                                                         const uint64_t* __restrict__ csum,
+                                                         uint32_t* __restrict__ out) {
+  constexpr int kWaves = kBlock / kWave;
+  constexpr int kPass = kTile / kBlock;         // passes of 64 rows per wave
+  constexpr int RC = MAXC ? MAXC : 1;           // row words held in registers
+  __shared__ uint32_t s_wcnt[kWaves][kMaxParts];  // per wave: its rows of owner p so far
+  __shared__ uint64_t s_base[kMaxParts];          // output row of the tile's first row of owner p
+  __shared__ uint64_t s_dst[MAXC ? 1 : kTile];    // wide tables: output row of each tile row
+  const uint64_t ntiles = (rows + kTile - 1) / kTile;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint64_t lt = (uint64_t(1) << lane) - 1;  // lanes below this one
+  const int bits = nparts > 1 ? 32 - __clz(int(nparts - 1)) : 0;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (uint32_t p = lane; p < nparts; p += kWave) s_wcnt[w][p] = 0;
+    for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) {
+      const uint64_t i = uint64_t(p) * ntiles + t;
+      s_base[p] = hist[i] + csum[i / kPartChunk];
+    }
+    const uint32_t l0 = uint32_t(w) * (kPass * kWave) + lane;  // this lane's first row in the tile
+    const uint64_t r0 = t * kTile + l0;
+    uint32_t v[kPass][RC], own[kPass], rk[kPass];
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const uint64_t r = r0 + uint64_t(k) * kWave;
+      if (MAXC) {
+#pragma unroll
+        for (int c = 0; c < RC; ++c) v[k][c] = (r < rows && c < int(cols)) ? in[r * cols + c] : 0;
+      } else {
+        v[k][0] = r < rows ? in[r * cols + col] : 0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      uint32_t key = v[k][0];
+#pragma unroll
+      for (int c = 1; c < RC; ++c)
+        if (c == int(col)) key = v[k][c];
+      own[k] = key % nparts;
+    }
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const bool valid = r0 + uint64_t(k) * kWave < rows;
+      const uint32_t my = own[k];
+      uint64_t m = __ballot(valid);
+      for (int b = 0; b < bits; ++b) {
+        const bool on = (my >> b) & 1;
+        const uint64_t bb = __ballot(on);
+        m &= on ? bb : ~bb;
+      }
+      const uint32_t rank = __popcll(m & lt);
+      rk[k] = valid ? s_wcnt[w][my] + rank : 0;
+      __builtin_amdgcn_wave_barrier();  // every lane has read before the counts move on
+      if (valid && rank == 0) s_wcnt[w][my] = rk[k] + __popcll(m);  // the owner's first lane
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();  // every wave's owner counts of this tile are in s_wcnt
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const uint64_t r = r0 + uint64_t(k) * kWave;
+      if (r < rows) {
+        for (int v2 = 0; v2 < w; ++v2) rk[k] += s_wcnt[v2][own[k]];
+        const uint64_t d = s_base[own[k]] + rk[k];
+        if (MAXC) {
+#pragma unroll
+          for (int c = 0; c < RC; ++c)
+            if (c < int(cols)) out[d * cols + c] = v[k][c];
+        } else {
+          s_dst[l0 + k * kWave] = d;
+        }
+      }
+    }
+    if (!MAXC) {
+      __syncthreads();
+      const uint64_t tr = t * kTile;
+      const uint32_t nrow = uint32_t(rows - tr < uint64_t(kTile) ? rows - tr : uint64_t(kTile));
+      const uint32_t nword = nrow * cols;
+      const uint32_t* src = in + tr * cols;
+      for (uint32_t e0 = threadIdx.x; e0 < nword; e0 += 4 * kBlock) {
+        uint32_t x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t e = e0 + u * kBlock;
+          x[u] = e < nword ? src[e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t e = e0 + u * kBlock;
+          if (e < nword) {
+            const uint32_t j = e / cols;
+            out[s_dst[j] * cols + (e - j * cols)] = x[u];
+          }
+        }
+      }
+    }
+    __syncthreads();  // s_wcnt, s_base and s_dst are free for the next tile
+  }
+}
+
 // Per-table checksums (sum of row hashes) of non-empty tables whose rows are
 // numbered consecutively; a block's 256 rows span at most 256 tables.
 __global__ __launch_bounds__(kBlock) void k_checksum(const uint32_t* base, const CkDesc* d, uint32_t n,
@@ -1195,6 +1393,36 @@ void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t
                      base, d, n, total_rows, reinterpret_cast<unsigned long long*>(sums));
 }
 
+
+uint64_t partition_scratch_words(uint64_t rows, uint32_t nparts) {
+  const uint64_t n = uint64_t(nparts) * ((rows + kTile - 1) / kTile);
+  return n + (n + kPartChunk - 1) / kPartChunk + nparts + 1;
+}
+
+uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col, uint32_t nparts,
+                           uint64_t* scratch, uint32_t* out, hipStream_t s) {
+  const uint64_t ntiles = (rows + kTile - 1) / kTile;
+  const uint64_t n = uint64_t(nparts) * ntiles, nch = (n + kPartChunk - 1) / kPartChunk;
+  uint64_t* hist = scratch;
+  uint64_t* csum = hist + n;
+  uint64_t* heads = csum + nch;
+  const unsigned grid = grid_for(ntiles, 8192);
+  hipLaunchKernelGGL(k_part_hist, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, hist);
+  hipLaunchKernelGGL(k_part_scan_local, dim3(grid_for(nch, 8192)), dim3(kBlock), 0, s, hist, n, csum);
+  hipLaunchKernelGGL(k_part_scan_chunks, dim3(1), dim3(1024), 0, s, (const uint64_t*)hist, csum, nch, ntiles,
+                     nparts, heads);
+  const uint64_t* h = hist;
+  const uint64_t* c = csum;
+  if (cols <= 2)
+    hipLaunchKernelGGL(k_part_scatter<2>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+  else if (cols <= 4)
+    hipLaunchKernelGGL(k_part_scatter<4>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+  else if (cols <= 8)
+    hipLaunchKernelGGL(k_part_scatter<8>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+  else
+    hipLaunchKernelGGL(k_part_scatter<0>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+  return heads;
+}
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((n + 255) / 256, 4096)), dim3(256), 0, s, p, n, v);

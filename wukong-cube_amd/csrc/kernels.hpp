@@ -126,6 +126,19 @@ void launch_scan_queries(const StepArgs& a, hipStream_t s);
 void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
 void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s);
 void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
+// Fork-join dispatch: out = in's rows grouped by owner row[col] % nparts
+// (nparts <= kMaxParts), in row order within each owner.  hist needs
+// nparts * ceil(rows / kTile) entries; on return hist[p * ntiles] is owner
+// p's first row in out and *total the row count.
+constexpr uint32_t kMaxParts = 64;
+// Scratch words launch_partition needs: per-tile owner histogram, chunk
+// totals, and the nparts + 1 heads.
+uint64_t partition_scratch_words(uint64_t rows, uint32_t nparts);
+// Groups rows by owner in[r * cols + col] % nparts into out, keeping input
+// order within each owner.  Returns the device address, inside scratch, of
+// nparts + 1 words: each owner's first output row, then the row count.
+uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col, uint32_t nparts,
+                           uint64_t* scratch, uint32_t* out, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 
 // One non-empty table for the checksum kernel: `rows` rows of `cols`

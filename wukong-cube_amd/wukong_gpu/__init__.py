@@ -125,6 +125,7 @@ SIGNATURES = {
     "wk_dev_reset_kernel_stats": (C.c_int, [P]),
     "wk_dev_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32]),
     "wk_dev_set_profile": (C.c_int, [P, C.c_int32]),
+    "wk_table_partition": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.POINTER(P), U64P, U64P]),
     "wk_dev_sync": (C.c_int, [P]),
     "wk_dev_load_all": (C.c_int, [P]),
     "wk_seg_ensure": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
@@ -394,6 +395,16 @@ class Device:
 
     def set_tuning(self, probe_width, rows_per_wave):
         _check(lib().wk_dev_set_tuning(self._h, probe_width, rows_per_wave))
+
+    def partition(self, table, col, nparts):
+        """Rows of a device Table grouped by owner row[col] % nparts: (Table, sizes, heads)."""
+        h = P()
+        sizes = np.zeros(nparts, np.uint64)
+        heads = np.zeros(nparts, np.uint64)
+        _check(lib().wk_table_partition(self._h, table._h, col, nparts, C.byref(h),
+                                        sizes.ctypes.data_as(U64P), heads.ctypes.data_as(U64P)),
+               "wk_table_partition")
+        return Table(self, h), sizes, heads
 
     def set_profile(self, on):
         _check(lib().wk_dev_set_profile(self._h, 1 if on else 0))
