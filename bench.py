@@ -134,7 +134,7 @@ def run_single(dev, singles, args):
         table, v2c, rc = dev.query(pats, nv, fetch=True)
         single[name] = {"patterns": len(pats), "rows": int(table.shape[0]), "cols": int(table.shape[1]),
                         "latency_us_mean": float(np.mean(lat)), "latency_us_p50": float(np.median(lat)),
-                        "kernels": ks, "_table": table, "_pats": pats, "_nv": nv}
+                        "kernels": ks, "_table": table, "_pats": pats, "_nv": nv, "_v2c": v2c}
         log("%s: %d rows, %.0f us/query" % (name, table.shape[0], np.mean(lat)))
     return single
 
@@ -169,6 +169,46 @@ def run_partition(dev, single, args, owners=(8, 64)):
         log("split of %s (%d x %d) among %d owners: %.3f ms, %.0f GB/s" % (name, host.shape[0], host.shape[1], n,
                                                                           ms, gbps))
     tbl.free()
+    return res
+
+
+def run_final_process(dev, single, args, cpu_rows=2_000_000):
+    """final_process on the device over the largest single-query result:
+    SELECT DISTINCT of all its variables ORDER BY the last one DESC LIMIT
+    100, timed per call (sort, dedup, projection, output allocation).  The
+    oracle runs the same on the first cpu_rows rows, one thread."""
+    if not single:
+        return None
+    import oracle
+    name = max(single, key=lambda k: single[k]["rows"])
+    r = single[name]
+    host, v2c, nv = r["_table"], r["_v2c"], r["_nv"]
+    req = [-(i + 1) for i in range(nv) if v2c[i] >= 0]
+    kw = dict(distinct=True, orders=[(req[-1], True)], limit=100)
+    tbl = dev.upload(host)
+    for _ in range(args.warmup):
+        dev.final_process(tbl, v2c, req, **kw).free()
+    dev.sync()
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        out = dev.final_process(tbl, v2c, req, **kw)
+        lat.append((time.perf_counter() - t) * 1e3)
+        out.free()
+    tbl.free()
+    ms = float(np.median(lat))
+    res = {"table": name, "rows": int(host.shape[0]), "cols": int(host.shape[1]), "query": "DISTINCT, ORDER BY "
+           "last variable DESC, LIMIT 100", "ms_p50": ms, "rows_per_s": host.shape[0] / (ms * 1e-3)}
+    if not args.no_cpu:
+        sample = host[:cpu_rows]
+        t = time.perf_counter()
+        oracle.final_process(sample, v2c, req, **kw)
+        cs = time.perf_counter() - t
+        res["cpu_oracle"] = {"rows": int(sample.shape[0]), "s": cs, "rows_per_s": sample.shape[0] / cs,
+                             "threads": 1}
+    log("final_process of %s (%d rows): %.2f ms on the device%s" % (
+        name, host.shape[0], ms, "; oracle %.0f rows/s" % res["cpu_oracle"]["rows_per_s"] if "cpu_oracle" in res
+        else ""))
     return res
 
 
@@ -224,13 +264,15 @@ def main():
     single = {} if args.batch_only else run_single(dev, singles, args)
     roofline = roofline_of(single, args)
     partition = run_partition(dev, single, args)
+    final = run_final_process(dev, single, args)
     edges = sum(r["rows"] for r in single.values())
     if args.single_only:
         for r in single.values():
-            for k in ("_table", "_pats", "_nv"):
+            for k in ("_table", "_pats", "_nv", "_v2c"):
                 r.pop(k, None)
         print(json.dumps({"metric": "single-query kernels", "single_query": single, "roofline": roofline,
-                          "partition": partition, "config": cfg}))
+                          "partition": partition, "final_process": final,
+                          "config": cfg}))
         dev.close()
         return
 
@@ -372,7 +414,7 @@ def main():
         log("cpu oracle: %.0f q/s over %d queries; gate %s" % (n / cpu_s, n, gate))
 
     for r in single.values():
-        for k in ("_table", "_pats", "_nv"):
+        for k in ("_table", "_pats", "_nv", "_v2c"):
             r.pop(k, None)
     for r in serve.values():
         for k in ("_ck", "_st", "_n"):
@@ -387,7 +429,8 @@ def main():
         "per_query_qps": qps_perq, "batched_vs_per_query": qps_batched / qps_perq,
         "single_query": single,
         "edges_per_s": (edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values())) if single else None,
-        "serve": serve, "partition": partition, "roofline": roofline, "cpu_baseline": cpu, "parity_gate": gate,
+        "serve": serve, "partition": partition, "final_process": final, "roofline": roofline,
+        "cpu_baseline": cpu, "parity_gate": gate,
         "gstore": {"h2d_bytes": dev.stats()["h2d_bytes"], "segments_evicted": dev.stats()["segments_evicted"]},
     })
     dev.close()

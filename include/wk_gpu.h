@@ -210,6 +210,21 @@ int wk_table_free(wk_dev* dev, wk_table* t);
  * device's row scratch, or the output table cannot be allocated. */
 int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t nparts, wk_table** out,
                        uint64_t* sizes, uint64_t* heads);
+/* SPARQLEngine::final_process (core/engine/sparql.hpp:1419-1546) on a device
+ * table: DISTINCT (rows sorted on all columns as signed words, a row dropped
+ * when it equals its predecessor on every required variable), ORDER BY
+ * (order_vars, order_desc[i] != 0 for DESC; ids compared as unsigned, ties
+ * in full-row order), OFFSET, LIMIT (< 0: none), then projection onto
+ * `required`.  v2c[v] is the column of variable -(v + 1), -1 for none.  The
+ * reference's GPU agent skips this step (gpu/gpu_agent.hpp:220-224); its CPU
+ * engine orders by the ids' strings, which this library does not hold.  An
+ * empty table comes back empty with its columns, unchecked.
+ * WK_E_NO_REQUIRED_VAR: nreq == 0.  WK_E_VERTEX_INVALID: a variable has no
+ * column.  WK_E_TOO_MANY_VARS: nreq or norder > WK_MAX_VARS. */
+int wk_table_final_process(wk_dev* dev, const wk_table* in, const int32_t* v2c, int32_t nvars,
+                           const int32_t* required, uint32_t nreq, int32_t distinct, const int32_t* order_vars,
+                           const int32_t* order_desc, uint32_t norder, int64_t offset, int64_t limit,
+                           wk_table** out);
 
 /* --------------------------------------------------------------- patterns */
 

@@ -359,3 +359,71 @@ def test_partition_rejects_bad_arguments(univ10_dev):
         with pytest.raises(wg.WkError):
             univ10_dev.partition(dt, col, n)
     dt.free()
+
+
+# ------------------------------------------------------------ final_process
+
+def _fp_case(dev, t, v2c, required, **kw):
+    want = oracle.final_process(t, v2c, required, **kw)
+    dt = dev.upload(t)
+    got = dev.final_process(dt, v2c, required, **kw)
+    assert got.shape == want.shape, (got.shape, want.shape, kw)
+    if want.size:
+        assert np.array_equal(got.numpy(), want), kw
+    got.free()
+    dt.free()
+
+
+@pytest.mark.parametrize("rows,cols,hi", [(1, 1, 4), (60, 3, 5), (1024, 2, 40), (1025, 4, 3), (5000, 3, 1 << 32),
+                                          (70000, 5, 50)])
+def test_final_process_matches_oracle(univ10_dev, rows, cols, hi):
+    """Device final_process equals the oracle's row for row: projection,
+    DISTINCT (signed full-row sort, words above 2^31 included), ORDER BY
+    ASC/DESC on one and two keys with full-row tie order, OFFSET/LIMIT
+    clamping, on small value ranges so that duplicates and ties abound."""
+    rng = np.random.default_rng(rows + cols)
+    t = rng.integers(0, hi, size=(rows, cols), dtype=np.uint64).astype(np.uint32)
+    v2c = list(rng.permutation(cols)) + [-1]  # the last variable has no column
+    v2c = [int(x) for x in v2c]
+    allv = [-(i + 1) for i in range(cols)]
+    last = -cols
+    cases = [
+        dict(required=allv),
+        dict(required=[allv[0]], distinct=True),
+        dict(required=allv[::-1], distinct=True),
+        dict(required=[last, allv[0]], orders=[(last, True)]),
+        dict(required=allv, orders=[(allv[0], False), (last, True)], offset=3, limit=7),
+        dict(required=[allv[0]], distinct=True, orders=[(allv[0], True)], offset=1),
+        dict(required=allv, distinct=True, orders=[(last, False)], limit=0),
+        dict(required=allv, offset=rows + 5),
+        dict(required=allv, distinct=True, offset=2, limit=1 << 40),
+    ]
+    for c in cases:
+        req = c.pop("required")
+        _fp_case(univ10_dev, t, v2c, req, **c)
+
+
+def test_final_process_on_a_query_result(univ10_dev, univ10_oracle):
+    """DISTINCT + ORDER BY + LIMIT over a real lubm_q1-shaped result."""
+    pats, nv = Q.lubm_q1()
+    t, v2c, rc = univ10_dev.query(pats, nv)
+    assert rc == 0 and t.shape[0] > 0
+    vars_ = [-(i + 1) for i in range(nv) if v2c[i] >= 0]
+    _fp_case(univ10_dev, t, v2c, vars_[:2], distinct=True, orders=[(vars_[1], True), (vars_[0], False)], limit=100)
+    _fp_case(univ10_dev, t, v2c, vars_, orders=[(vars_[-1], False)])
+
+
+def test_final_process_empty_and_errors(univ10_dev):
+    e = univ10_dev.upload(np.zeros((0, 3), np.uint32))
+    out = univ10_dev.final_process(e, [0, 1, 2], [-1])
+    assert out.shape == (0, 3)  # an empty result is left as it is
+    out.free()
+    e.free()
+    dt = univ10_dev.upload(np.arange(12, dtype=np.uint32).reshape(4, 3))
+    for req, orders, code in [([], (), wg.WK_E_NO_REQUIRED_VAR), ([-4], (), wg.WK_E_VERTEX_INVALID),
+                              ([1], (), wg.WK_E_VERTEX_INVALID), ([-1], [(-4, False)], wg.WK_E_VERTEX_INVALID),
+                              ([-1] * 65, (), wg.WK_E_TOO_MANY_VARS)]:
+        with pytest.raises(wg.WkError) as ei:
+            univ10_dev.final_process(dt, [0, 1, 2, -1], req, orders=orders)
+        assert ei.value.code == code
+    dt.free()

@@ -426,7 +426,8 @@ int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t n
       delete t;
       return WK_E_NOMEM_BUDGET;
     }
-    const uint64_t* dh = launch_partition(in->d, rows, in->cols, col, nparts, dev->pfx, t->d, dev->stream);
+    const uint64_t* dh =
+        launch_partition(in->d, rows, in->cols, col, OwnerFn{nparts, 0, 0}, dev->pfx, t->d, dev->stream);
     std::vector<uint64_t> h(size_t(nparts) + 1);
     if (hip_failed(hipMemcpyAsync(h.data(), dh, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream)) ||
         hip_failed(hipStreamSynchronize(dev->stream))) {

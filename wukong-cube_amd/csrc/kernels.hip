@@ -1063,14 +1063,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
 // row's place is its owner's base, plus the owner's rows in earlier tiles,
 // plus its rank among the owner's rows earlier in its own tile.
 
-__device__ __forceinline__ uint32_t row_owner(const uint32_t* in, uint32_t cols, uint32_t col, uint32_t nparts,
-                                              uint64_t r) {
-  return in[r * cols + col] % nparts;
+__device__ __forceinline__ uint32_t owner_of(OwnerFn f, uint32_t x) { return f.mod ? x % f.mod : (x >> f.shift) & f.mask; }
+
+__device__ __forceinline__ uint32_t row_owner(const uint32_t* in, uint32_t cols, uint32_t col, OwnerFn f, uint64_t r) {
+  return owner_of(f, in[r * cols + col]);
 }
 
 // Per-tile owner counts, owner-major: hist[p * ntiles + t].
 __global__ __launch_bounds__(kBlock) void k_part_hist(const uint32_t* in, uint64_t rows, uint32_t cols,
-                                                      uint32_t col, uint32_t nparts, uint64_t* hist) {
+                                                      uint32_t col, OwnerFn f, uint64_t* hist) {
+  const uint32_t nparts = owner_count(f);
   __shared__ uint32_t s_h[kMaxParts];
   const uint64_t ntiles = (rows + kTile - 1) / kTile;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -1078,7 +1080,7 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const uint32_t* in, uint64
     __syncthreads();
     for (uint32_t k = 0; k < kTile / kBlock; ++k) {
       const uint64_t r = t * kTile + uint64_t(k) * kBlock + threadIdx.x;
-      if (r < rows) atomicAdd(&s_h[row_owner(in, cols, col, nparts, r)], 1u);
+      if (r < rows) atomicAdd(&s_h[row_owner(in, cols, col, f, r)], 1u);
     }
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < nparts; p += kBlock) hist[uint64_t(p) * ntiles + t] = s_h[p];
@@ -1152,7 +1154,7 @@ __global__ __launch_bounds__(1024) void k_part_scan_chunks(const uint64_t* hist,
 // tile word by word, consecutive lanes on consecutive words.
 template <int MAXC>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(const uint32_t* __restrict__ in, uint64_t rows,
-                                                         uint32_t cols, uint32_t col, uint32_t nparts,
+                                                         uint32_t cols, uint32_t col, OwnerFn f,
                                                          const uint64_t* __restrict__ hist,
                                                          const uint64_t* __restrict__ csum,
                                                          uint32_t* __restrict__ out) {
@@ -1165,6 +1167,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const uint32_t* __restr
   const uint64_t ntiles = (rows + kTile - 1) / kTile;
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const uint64_t lt = (uint64_t(1) << lane) - 1;  // lanes below this one
+  const uint32_t nparts = owner_count(f);
   const int bits = nparts > 1 ? 32 - __clz(int(nparts - 1)) : 0;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     for (uint32_t p = lane; p < nparts; p += kWave) s_wcnt[w][p] = 0;
@@ -1191,7 +1194,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const uint32_t* __restr
 #pragma unroll
       for (int c = 1; c < RC; ++c)
         if (c == int(col)) key = v[k][c];
-      own[k] = key % nparts;
+      own[k] = owner_of(f, key);
     }
 #pragma unroll
     for (int k = 0; k < kPass; ++k) {
@@ -1399,29 +1402,107 @@ uint64_t partition_scratch_words(uint64_t rows, uint32_t nparts) {
   return n + (n + kPartChunk - 1) / kPartChunk + nparts + 1;
 }
 
-uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col, uint32_t nparts,
+uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col, OwnerFn f,
                            uint64_t* scratch, uint32_t* out, hipStream_t s) {
+  const uint32_t nparts = owner_count(f);
   const uint64_t ntiles = (rows + kTile - 1) / kTile;
   const uint64_t n = uint64_t(nparts) * ntiles, nch = (n + kPartChunk - 1) / kPartChunk;
   uint64_t* hist = scratch;
   uint64_t* csum = hist + n;
   uint64_t* heads = csum + nch;
   const unsigned grid = grid_for(ntiles, 8192);
-  hipLaunchKernelGGL(k_part_hist, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, hist);
+  hipLaunchKernelGGL(k_part_hist, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, hist);
   hipLaunchKernelGGL(k_part_scan_local, dim3(grid_for(nch, 8192)), dim3(kBlock), 0, s, hist, n, csum);
   hipLaunchKernelGGL(k_part_scan_chunks, dim3(1), dim3(1024), 0, s, (const uint64_t*)hist, csum, nch, ntiles,
                      nparts, heads);
   const uint64_t* h = hist;
   const uint64_t* c = csum;
   if (cols <= 2)
-    hipLaunchKernelGGL(k_part_scatter<2>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+    hipLaunchKernelGGL(k_part_scatter<2>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   else if (cols <= 4)
-    hipLaunchKernelGGL(k_part_scatter<4>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+    hipLaunchKernelGGL(k_part_scatter<4>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   else if (cols <= 8)
-    hipLaunchKernelGGL(k_part_scatter<8>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+    hipLaunchKernelGGL(k_part_scatter<8>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   else
-    hipLaunchKernelGGL(k_part_scatter<0>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, nparts, h, c, out);
+    hipLaunchKernelGGL(k_part_scatter<0>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   return heads;
+}
+
+// ------------------------------------------------------------ final_process
+// SPARQLEngine::final_process (core/engine/sparql.hpp:1419-1546) on the
+// device.  Rows are sorted as a permutation: (key, row) pairs go through a
+// stable LSD radix sort, one 32-bit key column at a time, 6 bits per stable
+// split.  The pairs' second word is the row index into the input table.
+
+__device__ __forceinline__ uint32_t sort_key(uint32_t x, uint32_t mode) {
+  return mode == kKeySigned ? x ^ 0x80000000u : mode == kKeyDesc ? ~x : x;
+}
+
+// pairs[i] = (key of row idx_i at column c, idx_i), idx_i = prev ? prev[2i + 1]
+// : i.  prev may equal pairs: thread i reads and writes only pair i.
+__global__ __launch_bounds__(kBlock) void k_fp_keys(const uint32_t* table, uint32_t cols, uint32_t c, uint32_t mode,
+                                                    const uint32_t* prev, uint64_t n, uint32_t* pairs) {
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+    const uint32_t r = prev ? prev[2 * i + 1] : uint32_t(i);
+    pairs[2 * i] = sort_key(table[uint64_t(r) * cols + c], mode);
+    pairs[2 * i + 1] = r;
+  }
+}
+
+// DISTINCT over rows in sorted order: flag 0 keeps a row (the first, or one
+// that differs from its predecessor on a required column), 1 drops it.
+// Equality on the required columns is transitive, so comparing with the
+// predecessor is comparing with the last kept row, as the reference does.
+__global__ __launch_bounds__(kBlock) void k_fp_dedup(const uint32_t* table, uint32_t cols, ColList req,
+                                                     const uint32_t* sorted, uint64_t n, uint32_t* pairs) {
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+    const uint32_t r = sorted[2 * i + 1];
+    uint32_t drop = 0;
+    if (i > 0) {
+      const uint32_t q = sorted[2 * i - 1];
+      drop = 1;
+      for (uint32_t j = 0; j < req.n; ++j)
+        if (table[uint64_t(r) * cols + req.col[j]] != table[uint64_t(q) * cols + req.col[j]]) drop = 0;
+    }
+    pairs[2 * i] = drop;
+    pairs[2 * i + 1] = r;
+  }
+}
+
+// out[i][j] = table[idx_(first + i)][req.col[j]], consecutive threads on
+// consecutive output words; idx_k = sorted ? sorted[2k + 1] : k.
+__global__ __launch_bounds__(kBlock) void k_fp_project(const uint32_t* table, uint32_t cols, ColList req,
+                                                       const uint32_t* sorted, uint64_t first, uint64_t n,
+                                                       uint32_t* out) {
+  const uint64_t total = n * req.n;
+  for (uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x; e < total; e += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t i = e / req.n;
+    const uint32_t j = uint32_t(e - i * req.n);
+    const uint64_t k = first + i;
+    const uint64_t r = sorted ? sorted[2 * k + 1] : k;
+    out[e] = table[r * cols + req.col[j]];
+  }
+}
+
+void launch_fp_keys(const uint32_t* table, uint32_t cols, uint32_t c, uint32_t mode, const uint32_t* prev, uint64_t n,
+                    uint32_t* pairs, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fp_keys, dim3(grid_for((n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s, table, cols, c,
+                     mode, prev, n, pairs);
+}
+
+void launch_fp_dedup(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted, uint64_t n,
+                     uint32_t* pairs, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fp_dedup, dim3(grid_for((n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s, table, cols,
+                     req, sorted, n, pairs);
+}
+
+void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted,
+                       uint64_t first, uint64_t n, uint32_t* out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fp_project, dim3(grid_for((n * req.n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
+                     table, cols, req, sorted, first, n, out);
 }
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {

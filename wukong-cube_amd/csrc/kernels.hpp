@@ -131,14 +131,37 @@ void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint,
 // nparts * ceil(rows / kTile) entries; on return hist[p * ntiles] is owner
 // p's first row in out and *total the row count.
 constexpr uint32_t kMaxParts = 64;
+// Owner of a row, from one of its words x: x % mod when mod > 0, else the
+// digit (x >> shift) & mask.
+struct OwnerFn {
+  uint32_t mod, shift, mask;
+};
+__host__ __device__ inline uint32_t owner_count(OwnerFn f) { return f.mod ? f.mod : f.mask + 1; }
+
 // Scratch words launch_partition needs: per-tile owner histogram, chunk
 // totals, and the nparts + 1 heads.
 uint64_t partition_scratch_words(uint64_t rows, uint32_t nparts);
-// Groups rows by owner in[r * cols + col] % nparts into out, keeping input
-// order within each owner.  Returns the device address, inside scratch, of
-// nparts + 1 words: each owner's first output row, then the row count.
-uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col, uint32_t nparts,
+// Groups rows by owner f(in[r * cols + col]) (at most kMaxParts owners) into
+// out, keeping input order within each owner.  Returns the device address,
+// inside scratch, of nparts + 1 words: each owner's first output row, then
+// the row count.
+uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col, OwnerFn f,
                            uint64_t* scratch, uint32_t* out, hipStream_t s);
+
+// final_process: key transforms, so that unsigned order on the key is the
+// wanted order on the word.
+enum : uint32_t { kKeySigned = 0, kKeyAsc = 1, kKeyDesc = 2 };
+constexpr uint32_t kSortDigitBits = 6;  // bits per stable split (kMaxParts owners)
+struct ColList {
+  uint32_t n;
+  uint32_t col[64];
+};
+void launch_fp_keys(const uint32_t* table, uint32_t cols, uint32_t c, uint32_t mode, const uint32_t* prev, uint64_t n,
+                    uint32_t* pairs, hipStream_t s);
+void launch_fp_dedup(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted, uint64_t n,
+                     uint32_t* pairs, hipStream_t s);
+void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted,
+                       uint64_t first, uint64_t n, uint32_t* out, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 
 // One non-empty table for the checksum kernel: `rows` rows of `cols`

@@ -108,6 +108,7 @@ _LIB = None
 P = C.c_void_p
 U32P = C.POINTER(C.c_uint32)
 U64P = C.POINTER(C.c_uint64)
+I32P = C.POINTER(C.c_int32)
 
 # name -> (restype, argtypes); every symbol declared in include/wk_gpu.h
 SIGNATURES = {
@@ -126,6 +127,8 @@ SIGNATURES = {
     "wk_dev_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32]),
     "wk_dev_set_profile": (C.c_int, [P, C.c_int32]),
     "wk_table_partition": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.POINTER(P), U64P, U64P]),
+    "wk_table_final_process": (C.c_int, [P, P, I32P, C.c_int32, I32P, C.c_uint32, C.c_int32, I32P, I32P,
+                                         C.c_uint32, C.c_int64, C.c_int64, C.POINTER(P)]),
     "wk_dev_sync": (C.c_int, [P]),
     "wk_dev_load_all": (C.c_int, [P]),
     "wk_seg_ensure": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int32]),
@@ -405,6 +408,19 @@ class Device:
                                         sizes.ctypes.data_as(U64P), heads.ctypes.data_as(U64P)),
                "wk_table_partition")
         return Table(self, h), sizes, heads
+
+    def final_process(self, table, v2c, required, distinct=False, orders=(), offset=0, limit=-1):
+        """SPARQLEngine::final_process of a device Table; orders: [(var, descending)].
+        Returns a new device Table of the required variables' columns."""
+        v = (C.c_int32 * max(1, len(v2c)))(*v2c)
+        req = (C.c_int32 * max(1, len(required)))(*required)
+        ov = (C.c_int32 * max(1, len(orders)))(*[o for o, _ in orders])
+        od = (C.c_int32 * max(1, len(orders)))(*[1 if d else 0 for _, d in orders])
+        h = P()
+        _check(lib().wk_table_final_process(self._h, table._h, v, len(v2c), req, len(required),
+                                            1 if distinct else 0, ov, od, len(orders), offset, limit,
+                                            C.byref(h)), "wk_table_final_process")
+        return Table(self, h)
 
     def set_profile(self, on):
         _check(lib().wk_dev_set_profile(self._h, 1 if on else 0))
