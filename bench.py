@@ -66,6 +66,10 @@ def parse_args():
     ap.add_argument("--univ", type=int, default=2560)
     ap.add_argument("--pl-vertices", type=int, default=100_000_000)
     ap.add_argument("--pl-triples", type=int, default=1_200_000_000)
+    ap.add_argument("--ids-dir", type=str, default="",
+                    help="build the host store from a directory of Wukong ID-triple files (id_*); it must "
+                         "hold the graph of --univ/--seed (e.g. written by wukong_gpu.write_id_triples), "
+                         "since the oracle checks against the generated triples")
     ap.add_argument("--nq", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=None, help="default 1 (university), 7 (power law)")
     ap.add_argument("--gstore-budget-gb", type=float, default=0, help="0: the whole store")
@@ -246,7 +250,12 @@ def main():
 
     tri, singles, make_stream, desc, cfg = make_workload(args, wg, Q)
     t0 = time.time()
-    store = wg.Store(tri, pinned=True)
+    if args.ids_dir:
+        store, nread = wg.Store.from_dir(args.ids_dir, pinned=True)
+        cfg["ids_dir"] = {"triples_read": nread, "load_s": time.time() - t0}
+        log("host gstore loaded from %s in %.1fs (%d triples read)" % (args.ids_dir, time.time() - t0, nread))
+    else:
+        store = wg.Store(tri, pinned=True)
     info = store.info()
     log("host gstore built in %.1fs: %s" % (time.time() - t0, info))
     dev = wg.Device(store, gstore_budget=int(args.gstore_budget_gb * (1 << 30)),

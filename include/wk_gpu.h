@@ -129,6 +129,17 @@ typedef struct {
  * arrays of (s,p,o) uint32 triples; they may be the same array. */
 int wk_store_build(const uint32_t* pso, uint64_t n_pso, const uint32_t* pos, uint64_t n_pos,
                    const wk_store_opts* opts, wk_store** out);
+/* BaseLoader::load's normal triples (loader/base_loader.hpp:536-576) from a
+ * POSIX directory (loader/posix_loader.hpp:49-69): every "id_*" file, read
+ * as whitespace-separated "s p o" ids until the first read that fails (an
+ * incomplete last triple is dropped).  Server `server` of `nservers` keeps
+ * the triples with s % nservers == server as out-edges and those with
+ * o % nservers == server as in-edges (PARTITION, core/store/vertex.hpp:59),
+ * then builds its store as wk_store_build does.  `triples_read` (may be
+ * NULL) receives the number of triples read from all files.
+ * WK_E_FILE_NOT_FOUND: the directory or a file cannot be read. */
+int wk_store_load_dir(const char* dir, uint32_t server, uint32_t nservers, const wk_store_opts* opts,
+                      wk_store** out, uint64_t* triples_read);
 void wk_store_destroy(wk_store* st);
 int wk_store_info_get(const wk_store* st, wk_store_info* info);
 int wk_store_segment(const wk_store* st, uint32_t i, wk_seg_info* seg);

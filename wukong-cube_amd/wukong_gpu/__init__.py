@@ -113,6 +113,7 @@ I32P = C.POINTER(C.c_int32)
 # name -> (restype, argtypes); every symbol declared in include/wk_gpu.h
 SIGNATURES = {
     "wk_store_build": (C.c_int, [U32P, C.c_uint64, U32P, C.c_uint64, C.POINTER(StoreOpts), C.POINTER(P)]),
+    "wk_store_load_dir": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(StoreOpts), C.POINTER(P), U64P]),
     "wk_store_destroy": (None, [P]),
     "wk_store_info_get": (C.c_int, [P, C.POINTER(StoreInfo)]),
     "wk_store_segment": (C.c_int, [P, C.c_uint32, C.POINTER(SegInfo)]),
@@ -252,6 +253,18 @@ class Store:
         _check(lib().wk_store_build(pso_p, pso_a.size // 3, pos_p, pos_a.size // 3, C.byref(opts),
                                     C.byref(self._h)), "wk_store_build")
 
+    @classmethod
+    def from_dir(cls, path, server=0, nservers=1, pinned=False, load_factor_pct=0):
+        """Store of server `server` of `nservers` from a directory of Wukong
+        ID-triple files (id_*).  Returns (store, triples read from the files)."""
+        st = cls.__new__(cls)
+        st._h = P()
+        n = C.c_uint64()
+        opts = StoreOpts(1 if pinned else 0, load_factor_pct)
+        _check(lib().wk_store_load_dir(os.fsencode(path), server, nservers, C.byref(opts), C.byref(st._h),
+                                       C.byref(n)), "wk_store_load_dir")
+        return st, n.value
+
     def close(self):
         if self._h:
             lib().wk_store_destroy(self._h)
@@ -297,6 +310,37 @@ class Store:
         p, n = U32P(), C.c_uint64()
         _check(lib().wk_store_get_index(self._h, tpid, d, C.byref(p), C.byref(n)))
         return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+
+
+def write_id_triples(path, triples, nfiles=1, prefix="id_"):
+    """Write (s, p, o) triples as Wukong ID-triple text files
+    (datagen/README.md): one "s p o" line each, split over nfiles files
+    named <prefix>0.nt, <prefix>1.nt, ..."""
+    t = np.ascontiguousarray(triples, dtype=np.uint32).reshape(-1, 3)
+    os.makedirs(path, exist_ok=True)
+    for k, part in enumerate(np.array_split(t, max(1, nfiles))):
+        with open(os.path.join(path, "%s%d.nt" % (prefix, k)), "w") as f:
+            if len(part):
+                f.write("\n".join("%d\t%d\t%d" % tuple(r) for r in part.tolist()))
+                f.write("\n")
+
+
+def load_str_index(path):
+    """Name -> id of a dataset's str_index and str_normal files, read as
+    StringServer::load_normal_or_index_file does (whitespace-separated
+    "string id" pairs up to the first that fails; stringserver/string_server.hpp:214-225)."""
+    names = {}
+    for fname in ("str_index", "str_normal"):
+        fp = os.path.join(path, fname)
+        if not os.path.exists(fp):
+            continue
+        with open(fp) as f:
+            tok = f.read().split()
+        for i in range(0, len(tok) - 1, 2):
+            if not tok[i + 1].isdigit() or int(tok[i + 1]) >= 1 << 32:
+                break
+            names[tok[i]] = int(tok[i + 1])
+    return names
 
 
 def make_query(patterns, nvars):
