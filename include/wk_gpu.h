@@ -64,7 +64,8 @@ enum {
   WK_E_OVERFLOW = 106,        /* a count exceeds the 64-bit/28-bit fields    */
   WK_E_SEG_TOO_LARGE = 107,   /* one segment exceeds the gstore budget       */
   WK_E_STATE = 108,           /* call out of order (e.g. no result yet)      */
-  WK_E_TOO_MANY_VARS = 109    /* more variables than WK_MAX_VARS             */
+  WK_E_TOO_MANY_VARS = 109,   /* more variables than WK_MAX_VARS             */
+  WK_E_WIRE = 110             /* malformed archive, or a form not read here  */
 };
 
 #define WK_IN 0
@@ -319,6 +320,98 @@ int wk_serve_result_rows(wk_dev* dev, uint32_t i, uint32_t* dst, uint64_t cap_el
 /* Order-independent checksum of a host table (same function the serving
  * path uses): sum over rows of hash_u64 of a per-row fold. */
 uint64_t wk_table_checksum(const uint32_t* table, uint64_t rows, uint32_t cols);
+
+/* ------------------------------------------------------------------- wire */
+/* SPARQLQuery as the reference's proxy and agents exchange it: a Bundle
+ * string (Bundle::to_str / init, core/common/bundle.hpp:96-103,155-164) is
+ * the 4-byte req_type SPARQL_QUERY = 0 (core/common/type.hpp:52-55) followed
+ * by a boost binary_oarchive of the query (bundle.hpp:64-70; boost 1.67.0,
+ * deps/deps.sh:5) with the member order of core/sparql/query.hpp:1086-1276,
+ * every class object_serializable (query.hpp:1288-1296), TRDF_MODE off.
+ * Only BASIC pattern groups cross: an archive that holds FILTERs, OPTIONAL
+ * or UNION groups, or attribute results, is refused on decode with the
+ * reference's status code for them.  Ids and enums as in the reference:
+ * state SQ_PATTERN = 0 .. SQ_REPLY = 5, dev_type CPU = 0 / GPU = 1,
+ * job_type FULL_JOB = 0 / SPLIT_JOB = 1 (query.hpp:79-101); v2c_map holds
+ * Result's ext words, type << 16 | column, 0xffff for an unbound variable
+ * (query.hpp:60-69). */
+#define WK_SQ_PATTERN 0
+#define WK_SQ_REPLY 5
+#define WK_DEV_CPU 0
+#define WK_DEV_GPU 1
+#define WK_NO_RESULT 0xffff
+
+typedef struct {
+  int32_t subject, predicate, object, direction;
+  int32_t pred_type;          /* a char in the archive; SID_t = 0 */
+} wk_wire_pattern;
+
+typedef struct {
+  /* SPARQLQuery scalars, archive order (query.hpp:1210-1229) */
+  int32_t q_type, qid, pqid, pg_type, state, dev_type, job_type, priority;
+  int32_t mt_factor, mt_tid, pattern_step, local_var;
+  int32_t corun_enabled, corun_step, fetch_step, union_done, optional_step;
+  int32_t limit;
+  uint32_t offset;
+  int32_t distinct;
+  /* PatternGroup (query.hpp:1113-1132): patterns and optional_new_vars (a
+   * std::set, written ascending) */
+  const wk_wire_pattern* patterns;
+  uint32_t npatterns;
+  const int32_t* new_vars;
+  uint32_t nnew_vars;
+  /* orders (Order: variable id, descending) */
+  const int32_t* order_vars;
+  const int32_t* order_desc;
+  uint32_t norders;
+  /* Result (query.hpp:1155-1181) */
+  int32_t col_num, row_num, attr_col_num, status_code, blind, nvars;
+  const int32_t* required_vars;
+  uint32_t nrequired;
+  const int32_t* v2c_map;
+  uint32_t nv2c;
+  const uint8_t* optional_matched;  /* vector<bool>, one byte per row */
+  uint64_t noptional_matched;
+  const uint32_t* table;            /* result_table; written when row_num > 0 */
+  uint64_t ntable;
+  uint64_t gpu_nelems;              /* Result::gpu (query.hpp:311-362), USE_GPU builds */
+  int32_t gpu_col_num;
+} wk_sparql_msg;
+
+typedef struct {
+  uint32_t library_version;  /* the archive's boost library version; 0 -> 16 (boost 1.67) */
+  int32_t use_gpu;           /* 1: Result carries its GPUResult member (-DUSE_GPU builds) */
+} wk_wire_opts;
+
+/* Encode `m` as a Bundle string.  `len` receives the size; with dst NULL or
+ * cap too small nothing is written and WK_E_INVALID_ARG is returned after
+ * `len` is set (call once to size, once to write). */
+int wk_wire_encode(const wk_sparql_msg* m, const wk_wire_opts* o, uint8_t* dst, uint64_t cap, uint64_t* len);
+/* Decode a Bundle string.  On success *out points to a message whose arrays
+ * the library owns until wk_wire_free.  WK_E_WIRE: malformed or truncated
+ * bytes, another req_type, or a library version below 7.
+ * WK_E_UNKNOWN_FILTER / WK_E_UNSUPPORT_UNION / WK_E_ATTR_DISABLE: FILTERs,
+ * OPTIONAL or UNION groups, attribute results. */
+int wk_wire_decode(const uint8_t* src, uint64_t len, const wk_wire_opts* o, wk_sparql_msg** out);
+void wk_wire_free(wk_sparql_msg* m);
+
+/* GPUAgent::execute_sparql_query (gpu/gpu_agent.hpp:173-244) on one server,
+ * bundle in, bundle out.  The patterns from pattern_step run on the device:
+ * from the first pattern (index or constant start), or, when the request
+ * carries a history table, from that table uploaded as
+ * GPUEngine::load_result_buf does (gpu/gpu_engine.hpp:248-263).  Then, unlike
+ * the reference's GPU agent and as its CPU engine does, final_process
+ * (core/engine/sparql.hpp:1419-1546, skipped when blind or empty), and
+ * send_reply's fields (gpu_agent.hpp:115-125): state SQ_REPLY, FULL_JOB,
+ * shrink() when blind (rows and columns kept, table and required_vars
+ * dropped).  A qid of -1 gets the handle's next id.  A query this engine
+ * cannot run (not dev_type GPU, a reply, unsupported pattern, ...) still
+ * gets a reply, whose status_code says why, as the CPU engine reports
+ * errors.  The reply is kept on the handle: `reply_len` gets its size and
+ * wk_agent_reply copies it out.  Returns WK_E_WIRE for undecodable bytes
+ * and device errors (WK_E_HIP, WK_E_NOMEM_BUDGET) as other calls do. */
+int wk_agent_execute(wk_dev* dev, const uint8_t* req, uint64_t len, const wk_wire_opts* o, uint64_t* reply_len);
+int wk_agent_reply(wk_dev* dev, uint8_t* dst, uint64_t cap);
 
 /* ------------------------------------------------------------ generators */
 

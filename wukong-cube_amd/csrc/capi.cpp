@@ -678,6 +678,7 @@ const char* wk_strerror(int code) {
     case WK_E_SEG_TOO_LARGE: return "segment exceeds the gstore budget";
     case WK_E_STATE: return "call out of order";
     case WK_E_TOO_MANY_VARS: return "too many variables";
+    case WK_E_WIRE: return "malformed or unsupported query archive";
     default: return "unknown status";
   }
 }

@@ -133,6 +133,9 @@ struct wk_dev {
   uint32_t last_cols = 0;
   // last batch
   uint64_t batch_used = 0;
+  // last agent reply bundle, and the agent's query ids
+  std::vector<uint8_t> agent_reply;
+  int32_t agent_qid = 0;
 
   // served results kept for checking
   std::vector<std::vector<uint32_t>> served_rows;

@@ -29,7 +29,7 @@ STATUS = {
     13: "WK_E_FILE_NOT_FOUND", 100: "WK_E_INVALID_ARG", 101: "WK_E_INVALID_TRIPLE",
     102: "WK_E_NOMEM_HOST", 103: "WK_E_NOMEM_BUDGET", 104: "WK_E_HIP",
     105: "WK_E_NO_DEVICE", 106: "WK_E_OVERFLOW", 107: "WK_E_SEG_TOO_LARGE",
-    108: "WK_E_STATE", 109: "WK_E_TOO_MANY_VARS",
+    108: "WK_E_STATE", 109: "WK_E_TOO_MANY_VARS", 110: "WK_E_WIRE",
 }
 globals().update({v: k for k, v in STATUS.items()})
 
@@ -104,6 +104,36 @@ class ServeResult(C.Structure):
                 ("batches", C.c_uint64)]
 
 
+class WirePattern(C.Structure):
+    _fields_ = [("subject", C.c_int32), ("predicate", C.c_int32), ("object", C.c_int32),
+                ("direction", C.c_int32), ("pred_type", C.c_int32)]
+
+
+# scalar members of wk_sparql_msg, in struct order
+_MSG_HEAD = ("q_type", "qid", "pqid", "pg_type", "state", "dev_type", "job_type", "priority",
+             "mt_factor", "mt_tid", "pattern_step", "local_var", "corun_enabled", "corun_step",
+             "fetch_step", "union_done", "optional_step", "limit")
+_MSG_RESULT = ("col_num", "row_num", "attr_col_num", "status_code", "blind", "nvars")
+
+
+class SparqlMsg(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in _MSG_HEAD] + [("offset", C.c_uint32), ("distinct", C.c_int32)] +
+                [("patterns", C.POINTER(WirePattern)), ("npatterns", C.c_uint32),
+                 ("new_vars", C.POINTER(C.c_int32)), ("nnew_vars", C.c_uint32),
+                 ("order_vars", C.POINTER(C.c_int32)), ("order_desc", C.POINTER(C.c_int32)),
+                 ("norders", C.c_uint32)] +
+                [(n, C.c_int32) for n in _MSG_RESULT] +
+                [("required_vars", C.POINTER(C.c_int32)), ("nrequired", C.c_uint32),
+                 ("v2c_map", C.POINTER(C.c_int32)), ("nv2c", C.c_uint32),
+                 ("optional_matched", C.POINTER(C.c_uint8)), ("noptional_matched", C.c_uint64),
+                 ("table", C.POINTER(C.c_uint32)), ("ntable", C.c_uint64),
+                 ("gpu_nelems", C.c_uint64), ("gpu_col_num", C.c_int32)])
+
+
+class WireOpts(C.Structure):
+    _fields_ = [("library_version", C.c_uint32), ("use_gpu", C.c_int32)]
+
+
 _LIB = None
 P = C.c_void_p
 U32P = C.POINTER(C.c_uint32)
@@ -153,6 +183,13 @@ SIGNATURES = {
                                   C.POINTER(ServeOpts), C.POINTER(ServeResult)]),
     "wk_serve_result_rows": (C.c_int, [P, C.c_uint32, U32P, C.c_uint64, U64P, U32P]),
     "wk_table_checksum": (C.c_uint64, [U32P, C.c_uint64, C.c_uint32]),
+    "wk_wire_encode": (C.c_int, [C.POINTER(SparqlMsg), C.POINTER(WireOpts), C.POINTER(C.c_uint8), C.c_uint64,
+                                 U64P]),
+    "wk_wire_decode": (C.c_int, [C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(WireOpts),
+                                 C.POINTER(C.POINTER(SparqlMsg))]),
+    "wk_wire_free": (None, [C.POINTER(SparqlMsg)]),
+    "wk_agent_execute": (C.c_int, [P, C.POINTER(C.c_uint8), C.c_uint64, C.POINTER(WireOpts), U64P]),
+    "wk_agent_reply": (C.c_int, [P, C.POINTER(C.c_uint8), C.c_uint64]),
     "wk_gen_university": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(U32P), U64P]),
     "wk_univ_id": (C.c_uint32, [C.c_char_p]),
     "wk_gen_powerlaw": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
@@ -341,6 +378,73 @@ def load_str_index(path):
                 break
             names[tok[i]] = int(tok[i + 1])
     return names
+
+
+# ----------------------------------------------------------------- wire form
+
+def query_msg(patterns, nvars, required, dev_type=1, **kw):
+    """A SPARQLQuery request as the proxy sends it, as a dict: the members'
+    default values (core/sparql/query.hpp:710-750, 370-397), BASIC group,
+    `patterns` as (s, p, o, d) in plan order, `required` variable ids.
+    Keyword arguments override members (limit, offset, distinct, orders,
+    blind, ...)."""
+    m = {n: 0 for n in _MSG_HEAD + _MSG_RESULT}
+    m.update(qid=-1, pqid=-1, mt_factor=1, limit=-1, offset=0, distinct=0, dev_type=dev_type,
+             nvars=nvars, patterns=[tuple(p) + (0,) * (5 - len(p)) for p in patterns], new_vars=[], orders=[],
+             required_vars=list(required), v2c_map=[], optional_matched=[],
+             table=np.zeros(0, np.uint32), gpu_nelems=0, gpu_col_num=0)
+    m.update(kw)
+    return m
+
+
+def _wire_opts(use_gpu, library_version):
+    return WireOpts(library_version, 1 if use_gpu else 0)
+
+
+def wire_encode(m, use_gpu=False, library_version=0):
+    """Bundle string (type word + boost binary archive) of a message dict."""
+    pats = (WirePattern * max(1, len(m["patterns"])))(*[WirePattern(*p) for p in m["patterns"]])
+    arr = lambda xs: (C.c_int32 * max(1, len(xs)))(*xs)
+    nv, ov = arr(m["new_vars"]), arr([o for o, _ in m["orders"]])
+    od = arr([1 if d else 0 for _, d in m["orders"]])
+    req, v2c = arr(m["required_vars"]), arr(m["v2c_map"])
+    om = (C.c_uint8 * max(1, len(m["optional_matched"])))(*[1 if b else 0 for b in m["optional_matched"]])
+    tab = np.ascontiguousarray(m["table"], dtype=np.uint32).reshape(-1)
+    msg = SparqlMsg(*[int(m[n]) for n in _MSG_HEAD], int(m["offset"]), int(m["distinct"]),
+                    pats, len(m["patterns"]), nv, len(m["new_vars"]), ov, od, len(m["orders"]),
+                    *[int(m[n]) for n in _MSG_RESULT], req, len(m["required_vars"]), v2c,
+                    len(m["v2c_map"]), om, len(m["optional_matched"]), tab.ctypes.data_as(U32P),
+                    tab.size, int(m["gpu_nelems"]), int(m["gpu_col_num"]))
+    o = _wire_opts(use_gpu, library_version)
+    n = C.c_uint64()
+    lib().wk_wire_encode(C.byref(msg), C.byref(o), None, 0, C.byref(n))
+    buf = (C.c_uint8 * max(1, n.value))()
+    _check(lib().wk_wire_encode(C.byref(msg), C.byref(o), buf, n.value, C.byref(n)), "wk_wire_encode")
+    return bytes(buf[:n.value])
+
+
+def wire_decode(data, use_gpu=False, library_version=0):
+    """Message dict of a Bundle string; raises WkError (WK_E_WIRE, ...)."""
+    src = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data if data else b"\0")
+    o = _wire_opts(use_gpu, library_version)
+    h = C.POINTER(SparqlMsg)()
+    _check(lib().wk_wire_decode(src, len(data), C.byref(o), C.byref(h)), "wk_wire_decode")
+    try:
+        s = h.contents
+        m = {n: getattr(s, n) for n in _MSG_HEAD + _MSG_RESULT + ("offset", "distinct")}
+        m["patterns"] = [(p.subject, p.predicate, p.object, p.direction, p.pred_type)
+                         for p in s.patterns[:s.npatterns]]
+        m["new_vars"] = list(s.new_vars[:s.nnew_vars])
+        m["orders"] = list(zip(s.order_vars[:s.norders], [bool(d) for d in s.order_desc[:s.norders]]))
+        m["required_vars"] = list(s.required_vars[:s.nrequired])
+        m["v2c_map"] = list(s.v2c_map[:s.nv2c])
+        m["optional_matched"] = [bool(b) for b in s.optional_matched[:s.noptional_matched]]
+        m["table"] = (np.ctypeslib.as_array(s.table, (s.ntable,)).copy() if s.ntable
+                      else np.zeros(0, np.uint32))
+        m["gpu_nelems"], m["gpu_col_num"] = s.gpu_nelems, s.gpu_col_num
+        return m
+    finally:
+        lib().wk_wire_free(h)
 
 
 def make_query(patterns, nvars):
@@ -595,6 +699,16 @@ class Device:
                                      C.byref(res)), "wk_serve_replay")
         return {"latency_us": lat, "start_us": start, "rows": rows, "status": status, "checksum": ck,
                 "wall_s": res.wall_s, "batches": res.batches}
+
+    def agent_execute(self, bundle, use_gpu=False, library_version=0):
+        """GPUAgent::execute_sparql_query: request Bundle bytes in, reply Bundle bytes out."""
+        src = (C.c_uint8 * max(1, len(bundle))).from_buffer_copy(bundle if bundle else b"\0")
+        o = _wire_opts(use_gpu, library_version)
+        n = C.c_uint64()
+        _check(lib().wk_agent_execute(self._h, src, len(bundle), C.byref(o), C.byref(n)), "wk_agent_execute")
+        buf = (C.c_uint8 * max(1, n.value))()
+        _check(lib().wk_agent_reply(self._h, buf, n.value), "wk_agent_reply")
+        return bytes(buf[:n.value])
 
     def served_rows(self, i):
         r, c = C.c_uint64(), C.c_uint32()
