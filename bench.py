@@ -60,8 +60,8 @@ def measured_traffic(kernel, workload):
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=("university", "powerlaw"), default="university")
     ap.add_argument("--univ", type=int, default=2560)
     ap.add_argument("--pl-vertices", type=int, default=100_000_000)
@@ -334,6 +334,49 @@ def main():
     out["batch_host_ms"] = batch_host
     if batch_host:
         log("batch host phases (ms per batch): " + ", ".join("%s %.3f" % kv for kv in sorted(batch_host.items())))
+    # the same job with every round's result arena copied to pinned host
+    # memory after the round, as a caller that wants the rows on the host
+    # does: the PCIe-inclusive rate, reported beside the HBM-resident one
+    L = wg.lib()
+    hip = wg.C.CDLL("/opt/rocm/lib/libamdhip64.so")  # hipHostMalloc / hipHostFree for the pinned buffer
+    host_buf = [np.zeros(1, np.uint32)]
+    host_ptr = [None]
+
+    def fetch_arena():
+        used = wg.C.c_uint64()
+        wg._check(L.wk_batch_download(dev._h, None, 0, wg.C.byref(used)))
+        if host_buf[0].size < used.value:
+            if host_ptr[0]:
+                hip.hipHostFree(host_ptr[0])
+            n = int(used.value) * 2
+            ptr = wg.C.c_void_p()
+            assert hip.hipHostMalloc(wg.C.byref(ptr), wg.C.c_size_t(n * 4), 0) == 0
+            host_ptr[0] = ptr
+            host_buf[0] = np.ctypeslib.as_array(wg.C.cast(ptr, wg.U32P), (n,))
+        wg._check(L.wk_batch_download(dev._h, host_buf[0].ctypes.data_as(wg.U32P), host_buf[0].size,
+                                      wg.C.byref(used)))
+        return used.value * 4
+
+    def run_job_fetch():
+        nbytes = 0
+        for p_r, _ in chain:
+            dev.batch(p_r, use_graph=not args.no_graph, fetch=False)
+            nbytes += fetch_arena()
+        return nbytes
+
+    run_job_fetch()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        fetched = run_job_fetch()
+    dev.sync()
+    fetch_s = (time.perf_counter() - t) / args.steps
+    out["pcie_inclusive"] = {"queries_per_s": args.nq / fetch_s, "ms_per_step": fetch_s * 1e3,
+                             "result_bytes_per_step": fetched,
+                             "note": "each round's result arena copied to pinned host memory"}
+    log("batched with results copied to the host: %.3f ms per job (%.1f MB) -> %.0f q/s"
+        % (fetch_s * 1e3, fetched / 1e6, args.nq / fetch_s))
+    host_buf[0] = None
+    hip.hipHostFree(host_ptr[0])
     ok_b = st_b == 0
     qps_batched = args.nq / batch_s
     log("batched: %.3f ms per %d queries in %d round(s) -> %.0f q/s (%d ok, %d over the table budget, "
@@ -354,6 +397,25 @@ def main():
     pq_s = (time.perf_counter() - t) / npq
     qps_perq = 1.0 / pq_s
     log("per-query: %.1f us/query -> %.0f q/s" % (pq_s * 1e6, qps_perq))
+
+    # the same sample through the wire boundary: request bundles (boost
+    # binary archives, as the proxy sends them) in, reply bundles with the
+    # projected rows out, one query at a time
+    bundles = [wg.wire_encode(wg.query_msg(p, nv, [-(i + 1) for i in range(nv)])) for p, nv in qs[:npq]]
+    t = time.perf_counter()
+    replies = [dev.agent_execute(b) for b in bundles]
+    ag_s = (time.perf_counter() - t) / npq
+    ag_rows = 0
+    for i, r in enumerate(replies):
+        m = wg.wire_decode(r)
+        want = int(rows_b[i]) if st_b[i] == 0 else None
+        assert m["status_code"] == 0 and (want is None or m["row_num"] == want), (i, m["status_code"])
+        ag_rows += m["row_num"]
+    out["agent"] = {"queries": npq, "queries_per_s": 1.0 / ag_s, "us_per_query": ag_s * 1e6,
+                    "reply_rows": ag_rows, "request_bytes_mean": float(np.mean([len(b) for b in bundles])),
+                    "note": "wk_agent_execute: decode, device run, final_process, download, encode"}
+    log("agent (bundle in, bundle out): %.1f us/query -> %.0f q/s, %d reply rows"
+        % (ag_s * 1e6, 1.0 / ag_s, ag_rows))
 
     # ---------------------------------------------------------------- serve
     serve = {}
