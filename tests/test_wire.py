@@ -162,3 +162,29 @@ def test_encode_sizes_and_errors():
     assert L.wk_wire_encode(wg.C.byref(msg), None, None, 0, wg.C.byref(n)) == wg.WK_E_INVALID_ARG
     assert L.wk_wire_encode(None, None, None, 0, wg.C.byref(n)) == wg.WK_E_INVALID_ARG
     assert len(wg.wire_encode(m)) == W.encoded_length(m)
+
+
+def test_mutated_archives_decode_or_fail_cleanly():
+    """Random byte changes, insertions and cuts of valid archives either
+    decode to a message whose own encoding decodes again, or are refused
+    with a status code."""
+    rng = np.random.default_rng(5)
+    codes = {wg.WK_E_WIRE, wg.WK_E_UNKNOWN_FILTER, wg.WK_E_UNSUPPORT_UNION, wg.WK_E_ATTR_DISABLE}
+    for i in range(3000):
+        use_gpu = bool(i & 1)
+        b = bytearray(W.encode(random_msg(rng), use_gpu=use_gpu))
+        kind = i % 3
+        at = int(rng.integers(0, len(b)))
+        if kind == 0:
+            b[at] = int(rng.integers(0, 256))
+        elif kind == 1:
+            b[at:at] = bytes(rng.integers(0, 256, size=int(rng.integers(1, 9)), dtype=np.uint8))
+        else:
+            b = b[:at] + b[at + int(rng.integers(1, 9)):]
+        try:
+            d = wg.wire_decode(bytes(b), use_gpu=use_gpu)
+        except wg.WkError as e:
+            assert e.code in codes, (i, e.code)
+            continue
+        again = wg.wire_encode(d, use_gpu=use_gpu)
+        assert wg.wire_decode(again, use_gpu=use_gpu) is not None
