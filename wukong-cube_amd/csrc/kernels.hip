@@ -406,6 +406,8 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
   static_assert(kTile % NT == 0 && NT % W == 0, "probe tile shape");
   if (a.st->status) return;
   __shared__ uint64_t s_rs[NT];
+  __shared__ ProbeDesc s_pd[NT];
+  __shared__ uint64_t s_in[NT];
   const uint64_t R = step_rows(a);
   const uint64_t ntiles = (R + kTile - 1) / kTile;
   constexpr int G = NT / W;
@@ -466,13 +468,21 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
     } else {
       const uint32_t nqt = q1 - q0 + 1;
       const bool staged = nqt <= uint32_t(NT);
-      if (staged && threadIdx.x < nqt) s_rs[threadIdx.x] = a.row_start[q0 + threadIdx.x];
+      if (staged && threadIdx.x < nqt) {
+        // the tile's queries: row start, probe descriptor and input offset,
+        // so that a row's chain starts from LDS instead of two dependent
+        // global loads
+        const uint32_t q = q0 + threadIdx.x;
+        s_rs[threadIdx.x] = a.row_start[q];
+        s_pd[threadIdx.x] = read_probe_desc(a.desc + q);
+        s_in[threadIdx.x] = a.desc[q].in_off;
+      }
       __syncthreads();
       for (int pass = 0; pass < kPasses; ++pass) {
         const uint64_t g = tlo + uint64_t(pass) * G + grp;
         if (g < thi) {
           uint32_t c = 0, eo = 0;
-          uint32_t q;
+          // each branch probes by itself, so only values leave LDS or memory
           if (staged) {
             uint32_t l = 0, h = nqt;
             while (h - l > 1) {
@@ -480,14 +490,16 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
               if (s_rs[m] <= g) l = m;
               else h = m;
             }
-            q = q0 + l;
+            const ProbeDesc d = s_pd[l];
+            if (d.op != OP_SKIP && d.seg >= 0)
+              probe_row<W>(a, d, a.segs[d.seg], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
           } else {
-            q = query_in(a.row_start, q0, q1 + 1, g);
+            const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
+            const ProbeDesc d = read_probe_desc(a.desc + q);
+            if (d.op != OP_SKIP && d.seg >= 0)
+              probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
+                           eo);
           }
-          const ProbeDesc d = read_probe_desc(a.desc + q);
-          if (d.op != OP_SKIP && d.seg >= 0)
-            probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
-                         eo);
           if (gl == 0) {
             a.cnt[g] = c;
             a.eoff[g] = eo;
