@@ -408,6 +408,7 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
   __shared__ uint64_t s_rs[NT];
   __shared__ ProbeDesc s_pd[NT];
   __shared__ uint64_t s_in[NT];
+  __shared__ DevSeg s_sg[NT];
   const uint64_t R = step_rows(a);
   const uint64_t ntiles = (R + kTile - 1) / kTile;
   constexpr int G = NT / W;
@@ -469,13 +470,15 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
       const uint32_t nqt = q1 - q0 + 1;
       const bool staged = nqt <= uint32_t(NT);
       if (staged && threadIdx.x < nqt) {
-        // the tile's queries: row start, probe descriptor and input offset,
-        // so that a row's chain starts from LDS instead of two dependent
-        // global loads
+        // the tile's queries: row start, probe descriptor, input offset and
+        // segment, so that a row's chain starts from LDS instead of three
+        // dependent global loads
         const uint32_t q = q0 + threadIdx.x;
         s_rs[threadIdx.x] = a.row_start[q];
-        s_pd[threadIdx.x] = read_probe_desc(a.desc + q);
+        const ProbeDesc d = read_probe_desc(a.desc + q);
+        s_pd[threadIdx.x] = d;
         s_in[threadIdx.x] = a.desc[q].in_off;
+        if (d.op != OP_SKIP && d.seg >= 0) s_sg[threadIdx.x] = a.segs[d.seg];
       }
       __syncthreads();
       for (int pass = 0; pass < kPasses; ++pass) {
@@ -492,7 +495,7 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
             }
             const ProbeDesc d = s_pd[l];
             if (d.op != OP_SKIP && d.seg >= 0)
-              probe_row<W>(a, d, a.segs[d.seg], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
+              probe_row<W>(a, d, s_sg[l], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
           } else {
             const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
             const ProbeDesc d = read_probe_desc(a.desc + q);
