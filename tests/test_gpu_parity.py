@@ -158,6 +158,27 @@ def test_batch_mixed_random(univ10_oracle):
     dev.close()
 
 
+def test_batch_tiles_spanning_more_queries_than_a_block():
+    """3,000 queries, many of them empty or invalid, so that a 1,024-row
+    probe tile spans more than 1,024 queries (row starts searched in global
+    memory) as well as fewer (query descriptors and segments staged in LDS)."""
+    tri, info = random_graph(23)
+    orc = oracle.Store(tri)
+    dev = wg.Device(wg.Store(tri), table_budget=256 << 20)
+    rng = np.random.default_rng(29)
+    qs = [random_query(rng, info) for _ in range(3000)]
+    ok = 0
+    for (pats, nv), (t, v2c, rc) in zip(qs, dev.batch(qs, use_graph=True)):
+        want, wv2c, wrc = orc.query(pats, nv)
+        assert rc == wrc, pats
+        if rc == 0:
+            assert v2c == wv2c
+            assert same_rows(t, want), pats
+            ok += 1
+    assert 0 < ok < len(qs)  # both valid and invalid queries took part
+    dev.close()
+
+
 def test_batch_overflow_splits(univ10_store, univ10_oracle):
     """A batch whose step table exceeds the table capacity is split and still
     exact; a query that alone exceeds it reports WK_E_NOMEM_BUDGET."""
