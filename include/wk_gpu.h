@@ -26,6 +26,10 @@
  *   query   : GPUEngine::execute_one_pattern looped by
  *             GPUAgent::execute_sparql_query (src/gpu/gpu_engine.hpp:265-338;
  *             src/gpu/gpu_agent.hpp:173-246)
+ *   wire    : Bundle(const SPARQLQuery&) / Bundle::get_sparql_query and the
+ *             GPU agent's request-to-reply handling
+ *             (src/core/common/bundle.hpp:64-70,105-115;
+ *             src/gpu/gpu_agent.hpp:115-125,173-244)
  */
 #ifndef WK_GPU_H
 #define WK_GPU_H
