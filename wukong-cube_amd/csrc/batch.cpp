@@ -43,21 +43,12 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   if (nq == 0 || S == 0) return WK_OK;
   int rc = reserve_steps(dev, S, nq);
   if (rc) return rc;
-  std::vector<int> segs;
-  {
-    std::vector<uint8_t> seen(dev->seg.size(), 0);
-    for (uint32_t q = q0; q < q1; q++)
-      if (plans[q].status == WK_OK)
-        for (const auto& sd : plans[q].steps)
-          if (sd.seg >= 0 && !seen[size_t(sd.seg)]) {
-            seen[size_t(sd.seg)] = 1;
-            segs.push_back(sd.seg);
-          }
-  }
-  rc = ensure_segments(dev, segs);
-  if (rc) return rc;
-
-  // descriptors [step][query]: the query's own steps, then skips
+  // descriptors [step][query]: the query's own steps, then skips.  The same
+  // pass marks the segments the batch reads (one byte each, with atomic
+  // reads and writes), which are then made resident before anything is
+  // launched.
+  std::vector<uint8_t> seen(dev->seg.size(), 0);
+  uint8_t* const mark = seen.data();
   const size_t qc = dev->nq_cap;
   StepDesc skip{};
   skip.op = OP_SKIP;
@@ -69,6 +60,17 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     for (uint32_t k = 0; k < S; k++) {
       StepDesc* d = dev->h_desc + k * qc + i;
       if (k < ns) {
+        const int sg = p.steps[k].seg;
+        if (sg >= 0) {
+          // read first: once a mark is set the line stays shared, unwritten
+          uint8_t m;
+#pragma omp atomic read
+          m = mark[sg];
+          if (!m) {
+#pragma omp atomic write
+            mark[sg] = 1;
+          }
+        }
         *d = p.steps[k];
         d->final_step = (k + 1 == ns) ? 1 : 0;
         d->in_rows = k == 0 ? 1 : 0;
@@ -88,6 +90,12 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   }
   dev->h_rs[nq] = nvalid;
   t = host_phase(dev, "host:batch_fill", t);
+  std::vector<int> segs;
+  for (size_t i = 0; i < seen.size(); i++)
+    if (seen[i]) segs.push_back(int(i));
+  rc = ensure_segments(dev, segs);
+  if (rc) return rc;
+  t = host_phase(dev, "host:batch_segs", t);
   for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
   dev->h_state[0].arena_base = arena_base;
   auto fail = [&](int code) {

@@ -142,6 +142,7 @@ int evict(wk_dev* dev, int v) {
   dev->h_segs[v].resident = 0;
   dev->h_segs[v].kphys = kNoPhys;
   dev->h_segs[v].vphys = kNoPhys;
+  dev->copy_pending = true;
   WK_HIP(hipMemcpyAsync(dev->d_segs + v, dev->h_segs + v, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
   dev->stats.segments_evicted++;
@@ -172,6 +173,7 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
     if (rc) return rc;
   }
   // blocks being reused may still be read by queued kernels
+  dev->copy_pending = true;
   WK_HIP(hipEventRecord(dev->compute_done, dev->stream));
   WK_HIP(hipStreamWaitEvent(dev->copy_stream, dev->compute_done, 0));
   const SegMeta& m = dev->st->segs[size_t(si)];
@@ -235,8 +237,13 @@ int ensure_segments(wk_dev* dev, const std::vector<int>& segs) {
       return rc;
     }
   }
-  WK_HIP(hipEventRecord(dev->copy_done, dev->copy_stream));
-  WK_HIP(hipStreamWaitEvent(dev->stream, dev->copy_done, 0));
+  // the compute stream waits for the copy stream only when something was
+  // queued there since it last waited (a load, an eviction or a prefetch)
+  if (dev->copy_pending) {
+    WK_HIP(hipEventRecord(dev->copy_done, dev->copy_stream));
+    WK_HIP(hipStreamWaitEvent(dev->stream, dev->copy_done, 0));
+    dev->copy_pending = false;
+  }
   return WK_OK;
 }
 

@@ -94,6 +94,7 @@ struct wk_dev {
   uint64_t ktab_n = 0, vtab_n = 0;
   uint64_t tick = 0;
   hipEvent_t copy_done = nullptr, compute_done = nullptr;
+  bool copy_pending = false;  // copy-stream work not yet waited for by the compute stream
 
   // tables and scratch
   uint32_t* tbuf[2] = {nullptr, nullptr};
