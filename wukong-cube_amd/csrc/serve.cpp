@@ -84,6 +84,9 @@ struct Server {
     StepState* const hst = reinterpret_cast<StepState*>(hb + off_st);
     const uint64_t* const hsum = reinterpret_cast<const uint64_t*>(hb + off_ck);
     std::vector<int> segs;
+    // segments already listed, one byte each (a linear search of `segs` per
+    // query grew with in-flight queries times segments)
+    std::vector<uint8_t> listed(dev->seg.size(), 0);
     uint64_t R = 0;
     uint32_t nfinal = 0;
     for (uint32_t i = 0; i < n; i++) {
@@ -97,7 +100,10 @@ struct Server {
       nfinal += d.final_step;
       hrs[i] = R;
       R += d.in_rows;
-      if (d.seg >= 0 && std::find(segs.begin(), segs.end(), d.seg) == segs.end()) segs.push_back(d.seg);
+      if (d.seg >= 0 && !listed[size_t(d.seg)]) {
+        listed[size_t(d.seg)] = 1;
+        segs.push_back(d.seg);
+      }
     }
     hrs[n] = R;
     if (R > dev->rcap) {
