@@ -28,6 +28,39 @@ void launch_batch_steps(wk_dev* dev, uint32_t S, uint32_t nq, bool timed) {
 
 }  // namespace
 
+int split_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+                bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end) {
+  if (dev->opts.profile) dev->kstats["batch:splits"].launches++;
+  const uint32_t mid = q0 + (q1 - q0) / 2;
+  uint64_t end1 = arena_base;
+  int rc = run_batch(dev, plans, q0, mid, arena_base, use_graph, rows, offsets, status, &end1);
+  if (rc) return rc;
+  return run_batch(dev, plans, mid, q1, end1, use_graph, rows, offsets, status, arena_end);
+}
+
+int run_alone(wk_dev* dev, const Plan& plan, uint64_t arena_base, uint64_t* rows, uint64_t* offset, int32_t* status,
+              uint64_t* arena_end) {
+  *arena_end = arena_base;
+  *rows = 0;
+  *offset = 0;
+  if (plan.status || plan.steps.empty()) {
+    *status = plan.status;
+    return WK_OK;
+  }
+  uint64_t r = 0;
+  const int rc = run_query_to(dev, plan, dev->arena + arena_base, dev->arena_cap - std::min(arena_base, dev->arena_cap),
+                              &r);
+  if (rc == WK_E_HIP || rc == WK_E_NO_DEVICE) return rc;
+  *status = rc;
+  if (rc == WK_OK) {
+    *rows = r;
+    *offset = arena_base;
+    *arena_end = arena_base + r * plan.cols;
+  }
+  dev->stats.queries_run++;
+  return WK_OK;
+}
+
 int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
               bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end) {
   const uint32_t nq = q1 - q0;
@@ -94,6 +127,12 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   for (size_t i = 0; i < seen.size(); i++)
     if (seen[i]) segs.push_back(int(i));
   rc = ensure_segments(dev, segs);
+  if (rc == WK_E_NOMEM_BUDGET || rc == WK_E_SEG_TOO_LARGE) {
+    // the segments of all the batch's steps do not fit the gstore budget
+    // together: run the halves one after the other, as for a table overflow
+    if (nq > 1) return split_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, arena_end);
+    return run_alone(dev, plans[q0], arena_base, &rows[q0], &offsets[q0], &status[q0], arena_end);
+  }
   if (rc) return rc;
   t = host_phase(dev, "host:batch_segs", t);
   for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
@@ -141,17 +180,10 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   bool overflow = false;
   for (uint32_t k = 0; k < S; k++) overflow |= dev->h_state[k].status != 0;
   if (overflow) {
-    if (nq == 1) {
-      status[q0] = WK_E_NOMEM_BUDGET;
-      return WK_OK;
-    }
     // an intermediate table overflowed: split the batch; halves reuse the
     // tables and append to the arena
-    const uint32_t mid = q0 + nq / 2;
-    uint64_t end1 = arena_base;
-    rc = run_batch(dev, plans, q0, mid, arena_base, use_graph, rows, offsets, status, &end1);
-    if (rc) return rc;
-    return run_batch(dev, plans, mid, q1, end1, use_graph, rows, offsets, status, arena_end);
+    if (nq == 1) return run_alone(dev, plans[q0], arena_base, &rows[q0], &offsets[q0], &status[q0], arena_end);
+    return split_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, arena_end);
   }
   uint64_t end = arena_base;
   for (uint32_t i = 0; i < nq; i++) {

@@ -3,7 +3,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <new>
+#include <vector>
 
 #include "device.hpp"
 
@@ -27,12 +29,13 @@ void free_dev(wk_dev* d) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : d->event_pool) (void)hipEventDestroy(e);
-  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs,   d->d_ktab,    d->d_vtab, d->tbuf[0], d->tbuf[1],
-                      d->arena,  d->cnt,    d->eoff,     d->pfx,       d->tile_sum, d->tile_row, d->d_desc,
-                      d->d_rs,   d->d_state, d->d_qres, d->d_qagg};
+  // tbuf[1] lies inside tbuf[0]'s allocation (one pool of two tables)
+  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs,   d->d_ktab,    d->d_vtab, d->tbuf[0], d->arena,
+                      d->cnt,    d->eoff,   d->pfx,      d->tile_sum,  d->tile_row, d->d_desc, d->d_rs,
+                      d->d_state, d->d_qres, d->d_qagg, d->d_win};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
-  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state, d->h_qres};
+  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state, d->h_qres, d->h_win};
   for (void* p : host_ptrs)
     if (p) (void)hipHostFree(p);
   if (d->copy_done) (void)hipEventDestroy(d->copy_done);
@@ -134,8 +137,12 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   d->rcap = d->tcap;
   d->tile_cap = d->rcap / kTile + 2;
   d->otile_cap = (d->tcap + d->arena_cap) / kOutTile + 2;
-  WK_HIP(hipMalloc(&d->tbuf[0], d->tcap * sizeof(uint32_t)));
-  WK_HIP(hipMalloc(&d->tbuf[1], d->tcap * sizeof(uint32_t)));
+  // the two ping-pong tables are one pool, which the per-query path uses as
+  // a stack of intermediate tables
+  WK_HIP(hipMalloc(&d->tbuf[0], 2 * d->tcap * sizeof(uint32_t)));
+  d->tbuf[1] = d->tbuf[0] + d->tcap;
+  WK_HIP(hipMalloc(&d->d_win, size_t(kWinGroup) * 3 * sizeof(uint64_t)));
+  WK_HIP(hipHostMalloc(&d->h_win, size_t(kWinGroup) * 3 * sizeof(uint64_t), hipHostMallocDefault));
   WK_HIP(hipMalloc(&d->arena, d->arena_cap * sizeof(uint32_t)));
   WK_HIP(hipMalloc(&d->cnt, d->rcap * sizeof(uint32_t)));
   WK_HIP(hipMalloc(&d->eoff, d->rcap * sizeof(uint32_t)));
@@ -446,20 +453,57 @@ int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t n
 
 // --------------------------------------------------------------- patterns
 
+// One pattern over a whole device table into a new table of its exact size:
+// slices of at most rcap input rows, each probed for its output size, then
+// probed again (unless it is the only one) and gathered at its offset.
 static int pattern_step(wk_dev* dev, StepDesc sd, const wk_table* in, wk_table** out) {
   if (!dev || !out) return WK_E_INVALID_ARG;
   *out = nullptr;
-  wk_table* t = new (std::nothrow) wk_table();
+  const uint64_t R = in ? in->rows : sd.in_rows;
+  const uint32_t* tin = in ? in->d : nullptr;
+  std::unique_ptr<wk_table> t(new (std::nothrow) wk_table());
   if (!t) return WK_E_NOMEM_HOST;
   t->cols = sd.out_cols;
-  uint64_t rows = 0;
-  int rc = run_single_step(dev, sd, in ? in->d : nullptr, &t->d, 0, true, &rows);
-  if (rc) {
-    delete t;
-    return rc;
+  std::vector<int> segs{sd.seg};
+  int rc = ensure_segments(dev, segs);
+  if (rc) return rc;
+  struct Unpin {
+    wk_dev* d;
+    std::vector<int>& s;
+    ~Unpin() { unpin_segments(d, s); }
+  } unpin{dev, segs};
+  const uint64_t nsl = R ? (R + dev->rcap - 1) / dev->rcap : 0;
+  std::vector<uint64_t> sl_rows(nsl, 0);
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < nsl; i++) {
+    const uint64_t r0 = i * dev->rcap, n = std::min(dev->rcap, R - r0);
+    rc = probe_scan(dev, sd, tin ? tin + r0 * sd.in_cols : nullptr, n, &sl_rows[i]);
+    if (rc) return rc;
+    total += sl_rows[i];
   }
-  t->rows = rows;
-  *out = t;
+  if (total) {
+    if (hipMalloc(&t->d, total * sd.out_cols * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      return WK_E_NOMEM_BUDGET;
+    }
+    uint64_t at = 0;
+    for (uint64_t i = 0; i < nsl; i++) {
+      if (!sl_rows[i]) continue;
+      const uint64_t r0 = i * dev->rcap, n = std::min(dev->rcap, R - r0);
+      const uint32_t* src = tin ? tin + r0 * sd.in_cols : nullptr;
+      uint64_t T = sl_rows[i];
+      if (nsl > 1) rc = probe_scan(dev, sd, src, n, &T);
+      if (!rc) rc = gather_window(dev, sd, src, n, t->d + at * sd.out_cols, 0, T);
+      if (rc) {
+        (void)hipStreamSynchronize(dev->stream);
+        (void)hipFree(t->d);
+        return rc;
+      }
+      at += T;
+    }
+  }
+  t->rows = total;
+  *out = t.release();
   return WK_OK;
 }
 
@@ -534,7 +578,7 @@ int wk_expand_k2c(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t 
 
 int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols, int32_t* v2c) {
   if (!dev || !q) return WK_E_INVALID_ARG;
-  dev->last_buf = -1;
+  dev->last_ptr = nullptr;
   dev->last_rows = 0;
   dev->last_cols = 0;
   Plan plan;
@@ -543,11 +587,12 @@ int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols,
   if (cols) *cols = 0;
   if (plan.status) return plan.status;
   uint64_t r = 0;
-  int buf = -1;
-  int rc = run_query_device(dev, plan, &r, &buf);
+  // the result goes to the result arena, in place of the last batch's
+  dev->batch_used = 0;
+  int rc = run_query_to(dev, plan, dev->arena, dev->arena_cap, &r);
   if (rc) return rc;
-  dev->last_buf = buf;
-  dev->last_rows = buf >= 0 ? r : 0;
+  dev->last_ptr = dev->arena;
+  dev->last_rows = r;
   dev->last_cols = plan.cols;
   if (rows) *rows = dev->last_rows;
   if (cols) *cols = plan.cols;
@@ -561,9 +606,9 @@ int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems) {
   const uint64_t n = dev->last_rows * dev->last_cols;
   if (n > cap_elems || (n && !dst)) return WK_E_INVALID_ARG;
   if (n == 0) return WK_OK;
-  if (dev->last_buf < 0) return WK_E_STATE;
+  if (!dev->last_ptr) return WK_E_STATE;
   WK_HIP(hipStreamSynchronize(dev->stream));
-  WK_HIP(hipMemcpy(dst, dev->tbuf[dev->last_buf], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  WK_HIP(hipMemcpy(dst, dev->last_ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   return WK_OK;
 }
 

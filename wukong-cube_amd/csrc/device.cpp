@@ -333,6 +333,8 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
   a.next_st = has_next ? dev->d_state + step + 1 : nullptr;
   a.qres = nullptr;
   a.qagg = dev->d_qagg;
+  a.win_lo = 0;
+  a.win_hi = ~uint64_t(0);
   return a;
 }
 
@@ -397,108 +399,223 @@ int flush_timings(wk_dev* dev) {
 
 // ------------------------------------------------------------ single step
 
-int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** tout, uint64_t tcap,
-                    bool alloc_out, uint64_t* out_rows, int prefetch_seg) {
-  *out_rows = 0;
-  if (sd.in_rows == 0) {
-    if (alloc_out) *tout = nullptr;
-    return WK_OK;
-  }
-  if (sd.in_rows > dev->rcap) return WK_E_NOMEM_BUDGET;
+int probe_scan(wk_dev* dev, const StepDesc& sd0, const uint32_t* tin, uint64_t R, uint64_t* total) {
+  *total = 0;
+  if (R == 0) return WK_OK;
+  if (R > dev->rcap) return WK_E_NOMEM_BUDGET;
   int rc = reserve_steps(dev, 1, 1);
   if (rc) return rc;
-  std::vector<int> segs{sd.seg};
-  rc = ensure_segments(dev, segs);
-  if (rc) return rc;
-  // the next pattern's segment streams in on the copy stream meanwhile
-  // (GPUEngineCuda pipeline, src/gpu/gpu_engine_cuda.hpp:143-151)
-  if (dev->opts.enable_pipeline && prefetch_seg >= 0) prefetch_segment(dev, prefetch_seg, segs);
+  StepDesc sd = sd0;
   sd.final_step = 0;
   sd.in_off = 0;
+  sd.in_rows = R;
   dev->h_desc[0] = sd;
   dev->h_rs[0] = 0;
-  dev->h_rs[1] = sd.in_rows;
+  dev->h_rs[1] = R;
   dev->h_state[0] = StepState{};
-  auto fail = [&](int code) {
-    unpin_segments(dev, segs);
-    return code;
-  };
   if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, sizeof(StepDesc), hipMemcpyHostToDevice, dev->stream)) ||
       hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, dev->stream)) ||
       hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, sizeof(StepState), hipMemcpyHostToDevice, dev->stream)))
-    return fail(WK_E_HIP);
-  StepArgs a = make_args(dev, 0, 1, tin, alloc_out ? nullptr : *tout, alloc_out ? ~uint64_t(0) : tcap, false);
-  const double R = double(sd.in_rows), C = double(sd.in_cols);
+    return WK_E_HIP;
+  // no output buffer yet: the host sizes it (or its windows) from the total
+  StepArgs a = make_args(dev, 0, 1, tin, nullptr, ~uint64_t(0), false);
+  const double Rd = double(R);
   hipEvent_t ev;
   // per-kernel HIP-event timing (opts.profile) with the algorithmic bytes of
   // each launch: key column 4 B + slot 16 B + cnt/eoff 8 B per row for the
   // probe, cnt 4 B + prefix 8 B per row for the row scan
-  const double ntiles = std::ceil(R / double(kTile));
+  const double ntiles = std::ceil(Rd / double(kTile));
   timed_begin(dev, &ev);
-  launch_probe(a, dev->opts.probe_width, sd.in_rows, dev->stream);
-  timed_end(dev, ev, "k_probe", 28 * R);
+  launch_probe(a, dev->opts.probe_width, R, dev->stream);
+  timed_end(dev, ev, "k_probe", 28 * Rd);
   timed_begin(dev, &ev);
   launch_scan_tiles(a, dev->stream);
   timed_end(dev, ev, "k_scan_tiles", 16 * ntiles);
   timed_begin(dev, &ev);
-  launch_scan_rows(a, sd.in_rows, dev->stream);
-  timed_end(dev, ev, "k_scan_rows", 12 * R);
+  launch_scan_rows(a, R, dev->stream);
+  timed_end(dev, ev, "k_scan_rows", 12 * Rd);
   timed_begin(dev, &ev);
   launch_scan_queries(a, dev->stream);
   timed_end(dev, ev, "k_scan_queries", 96);
   if (hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
       hip_failed(hipStreamSynchronize(dev->stream)))
-    return fail(WK_E_HIP);
-  const StepState stt = dev->h_state[0];
-  if (stt.status) return fail(WK_E_NOMEM_BUDGET);
-  const uint64_t total = stt.total_out_rows;
-  if (total > dev->otile_cap * uint64_t(kOutTile)) return fail(WK_E_NOMEM_BUDGET);
-  if (alloc_out) {
-    uint32_t* p = nullptr;
-    if (total) {
-      if (hipMalloc(&p, total * sd.out_cols * sizeof(uint32_t)) != hipSuccess) return fail(WK_E_NOMEM_BUDGET);
-    }
-    *tout = p;
-    a.tout = p;
-  }
-  if (total) {
-    const double E = double(total), Co = double(sd.out_cols);
-    const double bytes = sd.op == OP_EXPAND ? 4 * R * C + 4 * E + 4 * E * Co : 4 * R * C + 4 * E * Co;
-    if (dev->opts.rows_per_wave <= 0) {
-      timed_begin(dev, &ev);
-      launch_tile_rows(a, total, dev->stream);
-      timed_end(dev, ev, "k_tile_rows", 8 * std::ceil(E / kOutTile));
-    }
-    timed_begin(dev, &ev);
-    launch_gather_only(a, dev->opts.rows_per_wave, total, dev->stream);
-    timed_end(dev, ev, dev->opts.rows_per_wave <= 0 ? "k_gather" : "k_gather_rows", bytes);
-  }
-  unpin_segments(dev, segs);
-  *out_rows = total;
+    return WK_E_HIP;
+  if (dev->h_state[0].status) return WK_E_NOMEM_BUDGET;
+  *total = dev->h_state[0].total_out_rows;
   return WK_OK;
 }
 
-int run_query_device(wk_dev* dev, const Plan& plan, uint64_t* rows_out, int* buf_out) {
-  *rows_out = 0;
-  *buf_out = -1;
-  if (plan.status) return plan.status;
-  uint64_t rows = 1;  // the start pattern reads one virtual row
-  int cur = -1;
-  for (size_t k = 0; k < plan.steps.size(); k++) {
-    StepDesc sd = plan.steps[k];
-    sd.in_rows = (k == 0) ? 1 : rows;
-    if (sd.in_rows == 0) break;  // later patterns of an empty table stay empty
-    const int out = (cur == 0) ? 1 : 0;
-    uint32_t* tout = dev->tbuf[out];
-    const int next_seg = (k + 1 < plan.steps.size()) ? plan.steps[k + 1].seg : -1;
-    int rc = run_single_step(dev, sd, cur >= 0 ? dev->tbuf[cur] : nullptr, &tout, dev->tcap, false, &rows,
-                             next_seg);
-    if (rc) return rc;
-    cur = out;
+int gather_window(wk_dev* dev, const StepDesc& sd, const uint32_t* tin, uint64_t R, uint32_t* dst, uint64_t j0,
+                  uint64_t j1) {
+  if (j1 <= j0) return WK_OK;
+  StepArgs a = make_args(dev, 0, 1, tin, dst, ~uint64_t(0), false);
+  a.win_lo = j0;
+  a.win_hi = j1;
+  const double Rd = double(R), C = double(sd.in_cols), Co = double(sd.out_cols);
+  const uint64_t total = dev->h_state[0].total_out_rows;
+  const double E = double(j1 - j0);
+  // input rows read in proportion to the window's share of the step
+  const double Rw = total ? Rd * E / double(total) : 0;
+  const double bytes = sd.op == OP_EXPAND ? 4 * Rw * C + 4 * E + 4 * E * Co : 4 * Rw * C + 4 * E * Co;
+  hipEvent_t ev;
+  if (dev->opts.rows_per_wave <= 0) {
+    timed_begin(dev, &ev);
+    launch_tile_rows(a, j1 - j0, dev->stream);
+    timed_end(dev, ev, "k_tile_rows", 8 * std::ceil(E / kOutTile));
   }
-  if (plan.steps.empty()) rows = 0;
-  *rows_out = rows;
-  *buf_out = cur;
+  timed_begin(dev, &ev);
+  launch_gather_only(a, dev->opts.rows_per_wave, j1 - j0, dev->stream);
+  timed_end(dev, ev, dev->opts.rows_per_wave <= 0 ? "k_gather" : "k_gather_rows", bytes);
+  return WK_OK;
+}
+
+namespace {
+
+// Depth-first execution of one query with one host sync per pattern.
+// Intermediate tables live in the pool (the two table buffers, one
+// allocation of 2 * tcap elements); final rows go to a separate sink.  A
+// step whose output does not fit the room left in the pool runs in output
+// windows (chunks): each chunk is gathered, then carried through the
+// remaining steps, before the next chunk is re-probed from its input rows.
+// [0, pinned) of the pool holds the inputs of chunked steps above the
+// current one, which must outlive it.
+struct QueryExec {
+  wk_dev* dev;
+  const Plan& plan;
+  uint32_t* pool;
+  uint64_t pool_cap;
+  uint32_t* sink;
+  uint64_t sink_cap;
+  uint64_t sink_used = 0;  // elements
+  uint64_t rows = 0;
+
+  QueryExec(wk_dev* d, const Plan& p, uint32_t* dst, uint64_t cap)
+      : dev(d), plan(p), pool(d->tbuf[0]), pool_cap(2 * d->tcap), sink(dst), sink_cap(cap) {}
+
+  // Steps k.. over R rows at `in`, which occupy pool elements [in_lo, in_hi)
+  // (empty for the first step's virtual row).
+  int run(uint32_t k, const uint32_t* in, uint64_t R, uint64_t in_lo, uint64_t in_hi, uint64_t pinned) {
+    if (R == 0) return WK_OK;
+    const StepDesc& sd = plan.steps[k];
+    const uint32_t ic = sd.in_cols;
+    if (R > dev->rcap) {
+      // slices of at most rcap rows, each through the remaining steps; the
+      // whole input stays live meanwhile
+      for (uint64_t r0 = 0; r0 < R; r0 += dev->rcap) {
+        const int rc = run(k, in + r0 * ic, std::min(dev->rcap, R - r0), in_lo, in_hi, std::max(pinned, in_hi));
+        if (rc) return rc;
+      }
+      return WK_OK;
+    }
+    const bool fin = k + 1 == plan.steps.size();
+    const uint32_t oc = sd.out_cols;
+    std::vector<int> segs{sd.seg};
+    int rc = ensure_segments(dev, segs);
+    if (rc) return rc;
+    // the next pattern's segment streams in on the copy stream meanwhile
+    // (GPUEngineCuda pipeline, src/gpu/gpu_engine_cuda.hpp:143-151)
+    if (dev->opts.enable_pipeline && !fin) prefetch_segment(dev, plan.steps[k + 1].seg, segs);
+    uint64_t T = 0;
+    rc = probe_scan(dev, sd, in, R, &T);
+    if (rc || T == 0) {
+      unpin_segments(dev, segs);
+      return rc;
+    }
+    if (fin) {
+      if (T * oc > sink_cap - sink_used) {
+        unpin_segments(dev, segs);
+        return WK_E_NOMEM_BUDGET;  // the result itself does not fit its buffer
+      }
+      rc = gather_window(dev, sd, in, R, sink + sink_used, 0, T);
+      unpin_segments(dev, segs);
+      sink_used += T * oc;
+      rows += T;
+      return rc;
+    }
+    const uint64_t need = T * oc;
+    const uint64_t live = std::max(pinned, in_hi);
+    if (in_lo >= pinned + need && in_hi > in_lo) {
+      // into the dead space below the input (ping-pong)
+      rc = gather_window(dev, sd, in, R, pool + pinned, 0, T);
+      unpin_segments(dev, segs);
+      if (rc) return rc;
+      return run(k + 1, pool + pinned, T, pinned, pinned + need, pinned);
+    }
+    if (need <= (pool_cap - live) / 2) {
+      // after the input, leaving at least as much room for the steps below
+      rc = gather_window(dev, sd, in, R, pool + live, 0, T);
+      unpin_segments(dev, segs);
+      if (rc) return rc;
+      return run(k + 1, pool + live, T, live, live + need, pinned);
+    }
+    // chunks of C output rows after the (now pinned) input
+    const uint64_t C = ((pool_cap - live) / 2) / oc;
+    if (C == 0) {
+      unpin_segments(dev, segs);
+      return WK_E_NOMEM_BUDGET;
+    }
+    if (dev->opts.profile) dev->kstats["query:chunked_steps"].launches++;
+    uint32_t* const cbuf = pool + live;
+    const uint64_t nch = (T + C - 1) / C;
+    bool fresh = true;  // the scratch holds the probe of all R rows
+    for (uint64_t g0 = 0; g0 < nch; g0 += kWinGroup) {
+      const uint32_t n = uint32_t(std::min<uint64_t>(kWinGroup, nch - g0));
+      if (!fresh) {
+        // deeper steps reused the scratch: probe all R rows again
+        rc = ensure_segments(dev, segs);
+        if (rc) return rc;
+        rc = probe_scan(dev, sd, in, R, &T);
+        if (rc) {
+          unpin_segments(dev, segs);
+          return rc;
+        }
+        fresh = true;
+      }
+      // each chunk's first and last input row, and the prefix at the first
+      launch_window_rows(dev->pfx, R, g0 * C, C, T, n, dev->d_win, dev->stream);
+      if (hip_failed(hipMemcpyAsync(dev->h_win, dev->d_win, size_t(n) * 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                    dev->stream)) ||
+          hip_failed(hipStreamSynchronize(dev->stream))) {
+        unpin_segments(dev, segs);
+        return WK_E_HIP;
+      }
+      std::vector<uint64_t> win(dev->h_win, dev->h_win + size_t(n) * 3);
+      for (uint32_t i = 0; i < n; i++) {
+        const uint64_t j0 = (g0 + i) * C, j1 = std::min(T, j0 + C);
+        if (fresh) {
+          rc = gather_window(dev, sd, in, R, cbuf, j0, j1);
+          fresh = false;
+        } else {
+          // re-probe the chunk's input rows [ra, rb]: their output rows are
+          // the chunk's, shifted by the prefix before row ra
+          const uint64_t ra = win[3 * i], rb = win[3 * i + 1], pa = win[3 * i + 2];
+          rc = ensure_segments(dev, segs);
+          if (rc) return rc;
+          uint64_t Tc = 0;
+          rc = probe_scan(dev, sd, in + ra * ic, rb - ra + 1, &Tc);
+          if (!rc) rc = gather_window(dev, sd, in + ra * ic, rb - ra + 1, cbuf, j0 - pa, j1 - pa);
+        }
+        unpin_segments(dev, segs);
+        if (rc) return rc;
+        rc = run(k + 1, cbuf, j1 - j0, live, live + (j1 - j0) * oc, live);
+        if (rc) return rc;
+      }
+    }
+    return WK_OK;
+  }
+};
+
+}  // namespace
+
+int run_query_to(wk_dev* dev, const Plan& plan, uint32_t* dst, uint64_t cap, uint64_t* rows_out) {
+  *rows_out = 0;
+  if (plan.status) return plan.status;
+  if (plan.steps.empty()) return WK_OK;
+  QueryExec ex(dev, plan, dst, cap);
+  // the start pattern reads one virtual row
+  const int rc = ex.run(0, nullptr, 1, 0, 0, 0);
+  if (rc) return rc;
+  *rows_out = ex.rows;
   dev->stats.queries_run++;
   return WK_OK;
 }

@@ -128,8 +128,12 @@ struct wk_dev {
   // hipGraph cache for batches: key (steps, nq)
   std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs;
 
-  // last per-query result
-  int last_buf = -1;
+  // chunk boundaries of a step run in output windows: [kWinGroup][3]
+  uint64_t* d_win = nullptr;
+  uint64_t* h_win = nullptr;  // pinned mirror
+
+  // last per-query result (in the result arena)
+  const uint32_t* last_ptr = nullptr;
   uint64_t last_rows = 0;
   uint32_t last_cols = 0;
   // last batch
@@ -171,19 +175,35 @@ double host_phase(wk_dev* dev, const char* name, double t0);
 double host_now();
 int flush_timings(wk_dev* dev);
 
-// Run one step of a single query on the host-synchronous path; writes the
-// output into `tout` (capacity `tcap` elements) or allocates it when
-// `alloc_out` is set.  Returns rows in *out_rows.
-int run_single_step(wk_dev* dev, StepDesc sd, const uint32_t* tin, uint32_t** tout, uint64_t tcap,
-                    bool alloc_out, uint64_t* out_rows, int prefetch_seg = -1);
+// One step of one query over R (<= rcap) input rows at `tin`, step segment
+// resident: probe and scans, then the step's output rows come back to the
+// host (one sync) in *total.  The per-row prefix stays in the scratch for
+// gather_window until the next step reuses it.
+int probe_scan(wk_dev* dev, const StepDesc& sd, const uint32_t* tin, uint64_t R, uint64_t* total);
+// Output rows [j0, j1) of the step probe_scan ran last, written to `dst`
+// (row j at dst + (j - j0) * out_cols), in as many launches as the output
+// tile map needs.  No sync.
+int gather_window(wk_dev* dev, const StepDesc& sd, const uint32_t* tin, uint64_t R, uint32_t* dst, uint64_t j0,
+                  uint64_t j1);
 
-// Per-query execution, one host sync per pattern; the result table is left
-// in dev->tbuf[*buf] (rows * plan.cols elements).
-int run_query_device(wk_dev* dev, const Plan& plan, uint64_t* rows, int* buf);
+// Per-query execution, one host sync per pattern, depth first: the final
+// table goes to `dst` (capacity `cap` elements), intermediate tables to the
+// table pool; a step whose output does not fit the pool's room runs in
+// output chunks, each carried through the remaining steps.  WK_E_NOMEM_BUDGET
+// only when the result does not fit `dst` or one output row does not fit the
+// pool.
+int run_query_to(wk_dev* dev, const Plan& plan, uint32_t* dst, uint64_t cap, uint64_t* rows);
 
 // Batched execution of queries [q0, q1) of `plans`; results land in the
 // arena from `arena_base`.  Fills rows/offsets/status per query.
 int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
               bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end);
+// Queries [q0, q1) as two batches, one after the other.
+int split_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+                bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end);
+// One query of a batch by itself (per-query path, chunked as needed), its
+// result into the arena at arena_base.
+int run_alone(wk_dev* dev, const Plan& plan, uint64_t arena_base, uint64_t* rows, uint64_t* offset, int32_t* status,
+              uint64_t* arena_end);
 
 }  // namespace wk

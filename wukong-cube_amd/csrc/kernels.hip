@@ -102,6 +102,38 @@ __device__ __forceinline__ void block_find_queries(const uint64_t* rs, uint32_t 
   q1 = lo[1];
 }
 
+// Largest g in [0, n) with v[g] <= x, for two values x0 <= x1 at once, by
+// all NT threads of a block (v ascending, v[0] <= x0): block_find_queries
+// over a 64-bit index range.  Every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ void block_find_rows(const uint64_t* v, uint64_t n, uint64_t x0, uint64_t x1,
+                                                uint64_t& g0, uint64_t& g1) {
+  __shared__ uint32_t s_at[2][2];
+  uint64_t lo[2] = {0, 0}, len[2] = {n, n};
+  const uint64_t x[2] = {x0, x1};
+  for (int round = 0; len[0] > 1 || len[1] > 1; ++round) {
+    uint64_t st[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      st[k] = (len[k] + NT - 1) / NT;
+      const uint64_t o = uint64_t(threadIdx.x) * st[k];
+      if (len[k] > 1 && o < len[k] && v[lo[k] + o] <= x[k] && (o + st[k] >= len[k] || v[lo[k] + o + st[k]] > x[k]))
+        s_at[round & 1][k] = threadIdx.x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (len[k] > 1) {
+        const uint64_t o = uint64_t(s_at[round & 1][k]) * st[k];
+        lo[k] += o;
+        len[k] = (st[k] < len[k] - o) ? st[k] : len[k] - o;
+      }
+    }
+  }
+  g0 = lo[0];
+  g1 = lo[1];
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T x) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -132,6 +164,26 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* total)
   if (total) *total = wsum[NW - 1];
   __syncthreads();
   return wbase + x - v;
+}
+
+// Maximum over the block; every thread must call it.
+template <int NT>
+__device__ __forceinline__ uint64_t block_max(uint64_t v) {
+  constexpr int NW = NT / kWave;
+  __shared__ uint64_t wmax[NW];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const uint64_t y = __shfl_xor((unsigned long long)v, o, kWave);
+    v = v > y ? v : y;
+  }
+  if (lane == 0) wmax[w] = v;
+  __syncthreads();
+  uint64_t m = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) m = m > wmax[i] ? m : wmax[i];
+  __syncthreads();
+  return m;
 }
 
 template <int NT>
@@ -550,6 +602,27 @@ __global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {
   }
 }
 
+// End of the output rows the gather writes in this launch (the step's total,
+// or the host's window end), and the number of output tiles from win_lo.
+__device__ __forceinline__ uint64_t out_end(const StepArgs& a) {
+  const uint64_t t = a.st->total_out_rows;
+  return t < a.win_hi ? t : a.win_hi;
+}
+
+// Output tiles of the launch.  The tile -> row map covers the first
+// otile_cap of them (enough for every row that fits the tables and the
+// arena); tiles past it, which only a step with dropped final tables has,
+// find their rows themselves.
+__device__ __forceinline__ uint64_t out_tiles(const StepArgs& a) {
+  const uint64_t e = out_end(a);
+  return e > a.win_lo ? (e - a.win_lo + kOutTile - 1) / kOutTile : 0;
+}
+
+__device__ __forceinline__ uint64_t mapped_tiles(const StepArgs& a) {
+  const uint64_t n = out_tiles(a);
+  return n < a.otile_cap ? n : a.otile_cap;
+}
+
 // Output tile -> source row of its first output row: the largest g with
 // pfx[g] <= t * kOutTile (a zero-count row can never be that largest g, as
 // the row after it has the same prefix).  One thread per output tile, so a
@@ -557,10 +630,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {
 __global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
   if (a.st->status) return;
   const uint64_t R = step_rows(a);
-  uint64_t nt = (a.st->total_out_rows + kOutTile - 1) / kOutTile;
-  if (nt > a.otile_cap) nt = a.otile_cap;
+  const uint64_t nt = mapped_tiles(a);
   for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < nt; t += uint64_t(gridDim.x) * kBlock) {
-    const uint64_t j = t * kOutTile;
+    const uint64_t j = a.win_lo + t * kOutTile;
     uint64_t lo = 0, hi = R;
     while (hi - lo > 1) {
       const uint64_t mid = (lo + hi) >> 1;
@@ -569,6 +641,20 @@ __global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
     }
     a.tile_row[t] = lo;
   }
+}
+
+// Elements of the result arena left for this step's final tables.  A final
+// table larger than all of it can never fit: it takes no arena space in the
+// query scan, so it cannot push the tables after it out of the arena.
+__device__ __forceinline__ uint64_t arena_room(const StepArgs& a) {
+  const uint64_t base = a.st->arena_base;
+  return a.arena_cap > base ? a.arena_cap - base : 0;
+}
+
+// Status of a step once its output sizes are known: 1 when its intermediate
+// tables overflow the table capacity.
+__device__ __forceinline__ uint32_t step_status(const StepArgs& a, uint64_t carry_t) {
+  return carry_t > a.tcap ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
@@ -595,6 +681,7 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
   const bool cached = per <= kPer;
   uint64_t crows[kPer];
   uint32_t ccols[kPer];  // out_cols, or 0x80000000 | out_cols for a final step
+  const uint64_t room = arena_room(a);
   uint64_t et = 0, ea = 0, nr = 0;
   if (cached) {
     // every load is unconditional at a clamped, in-bounds index, so they all
@@ -623,7 +710,8 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
       crows[i] = rows;
       ccols[i] = in ? (dc[i] | (df[i] ? 0x80000000u : 0u)) : 0u;
       if (df[i]) {
-        ea += rows * dc[i];
+        const uint64_t el = rows * dc[i];
+        ea += el <= room ? el : 0;
       } else {
         et += rows * dc[i];
         nr += rows;
@@ -634,18 +722,20 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
       const uint64_t rows = rows_of(q);
       const StepDesc& d = a.desc[q];
       if (d.final_step) {
-        ea += rows * d.out_cols;
+        const uint64_t el = rows * d.out_cols;
+        ea += el <= room ? el : 0;
       } else {
         et += rows * d.out_cols;
         nr += rows;
       }
     }
   }
-  uint64_t tt = 0, ta = 0, tn = 0;
+  uint64_t tt = 0, tn = 0;
   const uint64_t tbase = a.st->tout_base, abase = a.st->arena_base;
   uint64_t xt = block_excl_scan<1024>(et, &tt) + tbase;
-  uint64_t xa = block_excl_scan<1024>(ea, &ta) + abase;
+  uint64_t xa = block_excl_scan<1024>(ea, nullptr) + abase;
   uint64_t xn = block_excl_scan<1024>(nr, &tn);
+  uint64_t aend = abase;  // end of this thread's last final table that fits
   for (uint32_t q = qlo; q < qhi; ++q) {
     uint64_t rows = 0;
     uint32_t oc = 0;
@@ -673,7 +763,8 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
       a.next_row_start[q] = xn;
     }
     if (fin) {
-      // a final table that does not fit the arena is dropped alone
+      // a final table that does not fit the arena is dropped alone; one
+      // larger than the whole room takes no arena space
       const bool fits = xa + el <= a.arena_cap;
       d.out_off = fits ? xa : kNoSpace;
       if (!fits) d.out_rows = 0;
@@ -681,16 +772,21 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
         a.qres[2 * uint64_t(q)] = fits ? rows : 0;
         a.qres[2 * uint64_t(q) + 1] = fits ? xa : kNoSpace;
       }
-      xa += el;
+      if (fits) aend = xa + el;
+      xa += el <= room ? el : 0;
     } else {
       d.out_off = xt;
       xt += el;
       xn += rows;
     }
   }
+  // the tables that fit are a prefix of the step's final tables in query
+  // order: the next step's tables follow the last of them, not the dropped
+  // ones, which take no room
+  const uint64_t carry_a = block_max<1024>(aend);
   if (threadIdx.x == 0) {
-    const uint64_t carry_t = tbase + tt, carry_a = abase + ta;
-    const uint32_t status = carry_t > a.tcap ? 1 : 0;
+    const uint64_t carry_t = tbase + tt;
+    const uint32_t status = step_status(a, carry_t);
     a.st->out_elems = tt;
     a.st->status = status;
     if (a.next) {
@@ -727,6 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_query_sums(StepArgs a, uint32_t nqb)
       const uint32_t oc = a.desc[q].out_cols;
       if (a.desc[q].final_step) {
         ea = rows * oc;
+        if (ea > arena_room(a)) ea = 0;
       } else {
         et = rows * oc;
         nr = rows;
@@ -742,11 +839,10 @@ __global__ __launch_bounds__(kBlock) void k_query_sums(StepArgs a, uint32_t nqb)
     }
     return;
   }
-  uint64_t nt = (a.st->total_out_rows + kOutTile - 1) / kOutTile;
-  if (nt > a.otile_cap) nt = a.otile_cap;
+  const uint64_t nt = mapped_tiles(a);
   const uint64_t nb = gridDim.x - nqb;
   for (uint64_t t = uint64_t(blockIdx.x - nqb) * kBlock + threadIdx.x; t < nt; t += nb * kBlock) {
-    const uint64_t j = t * kOutTile;
+    const uint64_t j = a.win_lo + t * kOutTile;
     uint64_t lo = 0, hi = R;
     while (hi - lo > 1) {
       const uint64_t mid = (lo + hi) >> 1;
@@ -809,7 +905,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
   const uint64_t el = rows * oc;
   const uint64_t tbase = a.st->tout_base, abase = a.st->arena_base;
   const uint64_t xt = block_excl_scan<kBlock>(fin ? 0 : el, nullptr) + v[0] + tbase;
-  const uint64_t xa = block_excl_scan<kBlock>(fin ? el : 0, nullptr) + v[1] + abase;
+  const uint64_t share = (fin && el <= arena_room(a)) ? el : 0;  // as k_query_sums counted it
+  const uint64_t xa = block_excl_scan<kBlock>(share, nullptr) + v[1] + abase;
   const uint64_t xn = block_excl_scan<kBlock>(fin ? 0 : rows, nullptr) + v[2];
   if (q < a.nq) {
     StepDesc& d = a.desc[q];
@@ -828,16 +925,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
       a.qres[2 * uint64_t(q) + 1] = fits ? xa : kNoSpace;
     }
   }
+  // the next step's arena base: the end of the last final table that fits
+  // (a prefix of the step's final tables), the largest over all blocks; the
+  // host uploads the later steps' bases as 0
+  const uint64_t bend = block_max<kBlock>((q < a.nq && fin && xa + el <= a.arena_cap) ? xa + el : abase);
+  if (threadIdx.x == 0 && a.next_st) atomicMax((unsigned long long*)&a.next_st->arena_base, (unsigned long long)bend);
   if (blockIdx.x == nqb - 1 && threadIdx.x == 0) {
-    const uint64_t tt = v[3], ta = v[4], tn = v[5];
-    const uint64_t carry_t = tbase + tt, carry_a = abase + ta;
-    const uint32_t status = carry_t > a.tcap ? 1 : 0;
+    const uint64_t tt = v[3], tn = v[5];
+    const uint64_t carry_t = tbase + tt;
+    const uint32_t status = step_status(a, carry_t);
     a.st->out_elems = tt;
     a.st->status = status;
     if (a.next) {
       a.next_row_start[a.nq] = tn;
       if (a.next_st) {
-        a.next_st->arena_base = carry_a;
         a.next_st->tout_base = 0;
         a.next_st->status = status;
       }
@@ -858,7 +959,7 @@ __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint
                                              uint64_t g, uint64_t j, uint64_t pg, uint32_t eo) {
   const StepDesc& d = a.desc[q];
   if (d.out_off == kNoSpace) return 0;  // final table dropped: no room in the arena
-  const uint64_t orow = j - p0;
+  const uint64_t orow = j - p0 - a.win_lo;
   uint32_t* dst = (d.final_step ? a.arena : a.tout) + d.out_off + orow * d.out_cols;
   const uint32_t* src = a.tin + d.in_off + (g - r0) * d.in_cols;
   const uint32_t ic = d.in_cols;
@@ -912,13 +1013,21 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   __shared__ uint64_t s_qp0[kQStage];  // and its first output row
   __shared__ unsigned long long s_ck[kQStage];  // and its checksum share from this tile
   const uint64_t R = step_rows(a);
-  const uint64_t total = a.st->total_out_rows;
-  uint64_t nt = (total + kOutTile - 1) / kOutTile;
-  if (nt > a.otile_cap) nt = a.otile_cap;
+  const uint64_t total = out_end(a);
+  const uint64_t nt = out_tiles(a);
   for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
-    const uint64_t j0 = t * kOutTile;
-    const uint64_t lo = a.tile_row[t];
-    const uint64_t hi = (t + 1 < nt) ? a.tile_row[t + 1] + 1 : R;
+    const uint64_t j0 = a.win_lo + t * kOutTile;
+    uint64_t lo, hi;
+    if (t + 1 < a.otile_cap) {
+      lo = a.tile_row[t];
+      hi = (t + 1 < nt) ? a.tile_row[t + 1] + 1 : R;
+    } else {
+      // past the tile map (dropped final tables before this tile): the rows
+      // holding this tile's first row and the next tile's
+      uint64_t r1;
+      block_find_rows<kBlock>(a.pfx, R, j0, j0 + kOutTile, lo, r1);
+      hi = (t + 1 < nt) ? r1 + 1 : R;
+    }
     const uint64_t nr = hi - lo;
     const bool staged = nr <= uint64_t(kStage);
     if (staged) {
@@ -976,7 +1085,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
           else h = m;
         }
         const uint64_t g = lo + l;
-        uint32_t* dst = dbase + (j - p0) * oc;
+        uint32_t* dst = dbase + (j - p0 - a.win_lo) * oc;
         const uint32_t* src = sbase + (g - r0) * ic;
         uint64_t rh = row_hash_init(oc);
         for (uint32_t c = 0; c < ic; ++c) {
@@ -1064,7 +1173,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
     const uint32_t q = find_query(a.row_start, a.nq, g);
     const uint64_t p = a.pfx[g];
     uint64_t hsum = 0;
-    for (uint32_t k = gl; k < c; k += LPR) hsum += emit_row<true>(a, q, g, p + k);
+    const uint64_t e = out_end(a);
+    for (uint32_t k = gl; k < c; k += LPR)
+      if (p + k >= a.win_lo && p + k < e) hsum += emit_row<true>(a, q, g, p + k);
     if (a.ck) add_checksum(a, q, hsum);
   }
 }
@@ -1308,6 +1419,29 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     p[i] = v;
 }
 
+// Largest g < R with pfx[g] <= j (pfx[0] = 0 <= j).
+__device__ __forceinline__ uint64_t row_of_output(const uint64_t* pfx, uint64_t R, uint64_t j) {
+  uint64_t lo = 0, hi = R;
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (pfx[mid] <= j) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C,
+                                                        uint64_t T, uint32_t n, uint64_t* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t j = j0 + uint64_t(i) * C;
+  const uint64_t je = (j + C < T ? j + C : T) - 1;
+  const uint64_t ra = row_of_output(pfx, R, j);
+  out[3 * uint64_t(i)] = ra;
+  out[3 * uint64_t(i) + 1] = row_of_output(pfx, R, je);
+  out[3 * uint64_t(i) + 2] = pfx[ra];
+}
+
 inline unsigned grid_for(uint64_t units, unsigned cap) {
   if (units == 0) return 1;
   return unsigned(units < cap ? units : cap);
@@ -1518,6 +1652,12 @@ void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req,
   if (n == 0) return;
   hipLaunchKernelGGL(k_fp_project, dim3(grid_for((n * req.n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
                      table, cols, req, sorted, first, n, out);
+}
+
+void launch_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C, uint64_t T, uint32_t n,
+                        uint64_t* out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_window_rows, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pfx, R, j0, C, T, n, out);
 }
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {

@@ -101,6 +101,12 @@ struct StepArgs {
   uint64_t* qres;          // [nq][2] rows and arena offset of each final table, or null
   uint64_t* qagg;          // [ceil(nq / kBlock)][3] per-block query sums (multi-block query scan)
   unsigned long long* ck;  // [nq] sums of the row hashes of final tables (zeroed before), or null
+  // Output window of the gather: it writes the step's output rows
+  // [win_lo, min(win_hi, total)), row j at (j - win_lo) past its table's
+  // start.  The batched and served paths use the whole step (0, ~0); the
+  // per-query path runs a step too large for its room window by window.
+  uint64_t win_lo;
+  uint64_t win_hi;
 };
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
@@ -163,6 +169,14 @@ void launch_fp_dedup(const uint32_t* table, uint32_t cols, const ColList& req, c
 void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted,
                        uint64_t first, uint64_t n, uint32_t* out, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+
+// Chunk boundaries of a step run in output windows of C rows: for chunk i
+// (output rows [j, min(T, j + C)) with j = j0 + i * C), out[3i] and
+// out[3i + 1] are the input rows holding its first and last output row and
+// out[3i + 2] the prefix at the first.  pfx holds the step's R + 1 prefixes.
+constexpr uint32_t kWinGroup = 4096;
+void launch_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C, uint64_t T, uint32_t n,
+                        uint64_t* out, hipStream_t s);
 
 // One non-empty table for the checksum kernel: `rows` rows of `cols`
 // elements at element offset `off`; rows of all tables numbered from `row0`.

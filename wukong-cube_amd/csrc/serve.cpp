@@ -118,6 +118,14 @@ struct Server {
       hst->tout_base = tout_base;
       hst->arena_base = arena_base;
       rc = ensure_segments(dev, segs);
+      if (rc == WK_E_NOMEM_BUDGET || rc == WK_E_SEG_TOO_LARGE) {
+        // the step's segments do not fit the gstore budget together
+        if (n == 1) {
+          failed.push_back(int(a));
+          return WK_OK;
+        }
+        goto split;
+      }
       if (rc) return rc;
       const uint32_t* tin = dev->tbuf[cur];
       uint32_t* tout = dev->tbuf[cur ^ 1];
@@ -221,34 +229,50 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   int cur = 0;
   uint64_t fp_used = 0;
   std::vector<uint32_t> inflight;
-  // queries whose step did not fit the tables beside the other in-flight
-  // queries' steps: restarted from their first pattern, alone
+  // queries whose step did not fit beside the other in-flight queries'
+  // steps (tables or segments): rerun from their first pattern, alone, on
+  // the per-query path, which runs steps too large for the tables in chunks
   std::vector<uint32_t> retry;
-  bool solo = false;  // a restarted query is in flight: admit nothing else
   std::vector<StepDesc> res;
   std::vector<int> failed;
   CkDesc* d_ck = nullptr;
   uint64_t* d_sum = nullptr;
-  if (opts->per_query) {
-    WK_HIP(hipMalloc(&d_ck, sizeof(CkDesc)));
-    WK_HIP(hipMalloc(&d_sum, sizeof(uint64_t)));
-  }
+  // one query by itself, pattern by pattern with a host sync per pattern,
+  // its result into the (then unused) result arena
+  auto run_alone = [&](uint32_t q) -> int {
+    uint64_t rows = 0;
+    const int st = run_query_to(dev, plans[q], dev->arena, dev->arena_cap, &rows);
+    if (st == WK_E_HIP) return st;
+    uint64_t ck = 0;
+    if (st == WK_OK && rows) {
+      CkDesc c{0, rows, 0, plans[q].cols, 0};
+      WK_HIP(hipMemcpyAsync(d_ck, &c, sizeof(c), hipMemcpyHostToDevice, dev->stream));
+      launch_fill_u64(d_sum, 1, 0, dev->stream);
+      launch_checksum(dev->arena, d_ck, 1, rows, d_sum, dev->stream);
+      WK_HIP(hipMemcpyAsync(&ck, d_sum, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream));
+      WK_HIP(hipStreamSynchronize(dev->stream));
+      const int kr = keep(q, dev->arena, rows, plans[q].cols);
+      if (kr) return kr;
+    } else if (opts->keep_results) {
+      dev->served_cols[q] = plans[q].cols;
+    }
+    finish(q, now_s(), st == WK_OK ? rows : 0, st, ck);
+    ndone++;
+    return WK_OK;
+  };
+  WK_HIP(hipMalloc(&d_ck, sizeof(CkDesc)));
+  WK_HIP(hipMalloc(&d_sum, sizeof(uint64_t)));
   int rc = WK_OK;
   while (ndone < nq && rc == WK_OK) {
     double now = now_s();
     if (!retry.empty() && inflight.empty()) {
       const uint32_t q = retry.front();
       retry.erase(retry.begin());
-      const double admit = qs[q].admit;
-      const uint32_t tries = qs[q].retries;
-      qs[q] = QState{};
-      qs[q].admit = admit;
-      qs[q].retries = tries;
-      fp_used += fp[q];
-      inflight.push_back(q);
-      solo = true;
+      batches++;
+      rc = run_alone(q);
+      continue;
     }
-    while (!solo && retry.empty() && next < nq && arrival_s[next] <= now && inflight.size() < max_inflight) {
+    while (retry.empty() && next < nq && arrival_s[next] <= now && inflight.size() < max_inflight) {
       const uint32_t q = next;
       if (plans[q].status || plans[q].steps.empty()) {
         qs[q].admit = now;
@@ -274,34 +298,11 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     }
     batches++;
     if (opts->per_query) {
-      // one query, pattern by pattern with a host sync per pattern
+      // the per-query baseline: one query at a time, FIFO
       const uint32_t q = inflight[0];
-      uint64_t rows = 0;
-      int buf = -1;
-      int st = run_query_device(dev, plans[q], &rows, &buf);
-      uint64_t ck = 0;
-      if (st == WK_OK && rows) {
-        CkDesc c{0, rows, 0, plans[q].cols, 0};
-        if (hip_failed(hipMemcpyAsync(d_ck, &c, sizeof(c), hipMemcpyHostToDevice, dev->stream))) {
-          rc = WK_E_HIP;
-          break;
-        }
-        launch_fill_u64(d_sum, 1, 0, dev->stream);
-        launch_checksum(dev->tbuf[buf], d_ck, 1, rows, d_sum, dev->stream);
-        if (hip_failed(hipMemcpyAsync(&ck, d_sum, sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream)) ||
-            hip_failed(hipStreamSynchronize(dev->stream))) {
-          rc = WK_E_HIP;
-          break;
-        }
-        rc = keep(q, dev->tbuf[buf], rows, plans[q].cols);
-      } else if (opts->keep_results) {
-        dev->served_cols[q] = plans[q].cols;
-      }
-      if (st != WK_OK) rows = 0;
-      finish(q, now_s(), rows, st, ck);
-      fp_used -= fp[q];
       inflight.clear();
-      ndone++;
+      fp_used -= fp[q];
+      rc = run_alone(q);
       continue;
     }
     res.assign(inflight.size(), StepDesc{});
@@ -316,16 +317,10 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     for (size_t i = 0; i < inflight.size(); i++) {
       const uint32_t q = inflight[i];
       if (std::find(failed.begin(), failed.end(), int(i)) != failed.end()) {
+        // rerun from its first pattern, alone, once the others have drained
         fp_used -= fp[q];
-        if (inflight.size() > 1 && qs[q].retries < 3) {
-          // it may fit alone: restart it from its first pattern later
-          qs[q].retries++;
-          retry.push_back(q);
-          continue;
-        }
-        finish(q, t, 0, WK_E_NOMEM_BUDGET, 0);
-        if (opts->keep_results) dev->served_cols[q] = plans[q].cols;
-        ndone++;
+        qs[q].retries++;
+        retry.push_back(q);
         continue;
       }
       const StepDesc& d = res[i];
@@ -345,7 +340,6 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
       }
     }
     inflight.swap(still);
-    if (inflight.empty()) solo = false;
     cur ^= 1;
   }
   if (d_ck) (void)hipFree(d_ck);

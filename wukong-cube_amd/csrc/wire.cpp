@@ -405,7 +405,7 @@ int wk_agent_execute(wk_dev* dev, const uint8_t* req, uint64_t len, const wk_wir
     uint32_t cols = 0;
     rc = wk_query_run(dev, &wq, &rows, &cols, v2c.data());
     if (rc == WK_OK) {
-      view.d = dev->last_buf >= 0 ? dev->tbuf[dev->last_buf] : nullptr;
+      view.d = const_cast<uint32_t*>(dev->last_ptr);
       view.rows = rows;
       view.cols = cols;
       result = &view;
