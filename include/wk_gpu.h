@@ -69,7 +69,8 @@ enum {
   WK_E_SEG_TOO_LARGE = 107,   /* one segment exceeds the gstore budget       */
   WK_E_STATE = 108,           /* call out of order (e.g. no result yet)      */
   WK_E_TOO_MANY_VARS = 109,   /* more variables than WK_MAX_VARS             */
-  WK_E_WIRE = 110             /* malformed archive, or a form not read here  */
+  WK_E_WIRE = 110,            /* malformed archive, or a form not read here  */
+  WK_E_SCRATCH_LIMIT = 111    /* input exceeds a fixed scratch area          */
 };
 
 #define WK_IN 0
@@ -226,6 +227,19 @@ int wk_table_free(wk_dev* dev, wk_table* t);
  * device's row scratch, or the output table cannot be allocated. */
 int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t nparts, wk_table** out,
                        uint64_t* sizes, uint64_t* heads);
+/* wk_table_partition with no allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid it).  `in` and `out` are the
+ * caller's device buffers of rows x cols uint32 (out must not overlap in);
+ * the owner x tile histogram lives in a fixed 8 MiB device array per GPU,
+ * set up by wk_dev_create, so the call makes no device or host allocation.
+ * nparts * ceil(rows / 1024) must stay below about 10^6 (64 owners over 16M
+ * rows, 8 owners over 128M rows), else WK_E_SCRATCH_LIMIT.  Calls that
+ * overlap on one GPU share that array and race.  Returns once the split is
+ * done (sizes and heads as wk_table_partition).  WK_E_INVALID_ARG: nparts
+ * outside 1..64, col >= cols, or null buffers for a non-empty table. */
+int wk_table_partition_noalloc(wk_dev* dev, const uint32_t* in, uint64_t rows, uint32_t cols, uint32_t col,
+                               uint32_t nparts, uint32_t* out, uint64_t* sizes, uint64_t* heads);
 /* SPARQLEngine::final_process (core/engine/sparql.hpp:1419-1546) on a device
  * table: DISTINCT (rows sorted on all columns as signed words, a row dropped
  * when it equals its predecessor on every required variable), ORDER BY

@@ -62,6 +62,9 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   if (o->device < 0 || o->device >= n) return WK_E_INVALID_ARG;
   d->device = o->device;
   WK_HIP(hipSetDevice(d->device));
+  // the no-allocation entry points' fixed scratch on this device
+  int src = noalloc_init(d->device);
+  if (src) return src;
   WK_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   WK_HIP(hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
   WK_HIP(hipEventCreateWithFlags(&d->copy_done, hipEventDisableTiming));
@@ -724,6 +727,7 @@ const char* wk_strerror(int code) {
     case WK_E_STATE: return "call out of order";
     case WK_E_TOO_MANY_VARS: return "too many variables";
     case WK_E_WIRE: return "malformed or unsupported query archive";
+    case WK_E_SCRATCH_LIMIT: return "input exceeds the fixed scratch area";
     default: return "unknown status";
   }
 }

@@ -160,6 +160,10 @@ int ensure_segments(wk_dev* dev, const std::vector<int>& segs);
 void unpin_segments(wk_dev* dev, const std::vector<int>& segs);
 int prefetch_segment(wk_dev* dev, int seg, const std::vector<int>& in_use);
 
+// Device address of the no-allocation entry points' fixed scratch on
+// `device` (the current device), looked up once per device.
+int noalloc_init(int device);
+
 // Capacity management for step descriptors.
 int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq);
 

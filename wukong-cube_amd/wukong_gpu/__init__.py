@@ -29,7 +29,7 @@ STATUS = {
     13: "WK_E_FILE_NOT_FOUND", 100: "WK_E_INVALID_ARG", 101: "WK_E_INVALID_TRIPLE",
     102: "WK_E_NOMEM_HOST", 103: "WK_E_NOMEM_BUDGET", 104: "WK_E_HIP",
     105: "WK_E_NO_DEVICE", 106: "WK_E_OVERFLOW", 107: "WK_E_SEG_TOO_LARGE",
-    108: "WK_E_STATE", 109: "WK_E_TOO_MANY_VARS", 110: "WK_E_WIRE",
+    108: "WK_E_STATE", 109: "WK_E_TOO_MANY_VARS", 110: "WK_E_WIRE", 111: "WK_E_SCRATCH_LIMIT",
 }
 globals().update({v: k for k, v in STATUS.items()})
 
@@ -158,6 +158,8 @@ SIGNATURES = {
     "wk_dev_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32]),
     "wk_dev_set_profile": (C.c_int, [P, C.c_int32]),
     "wk_table_partition": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.POINTER(P), U64P, U64P]),
+    "wk_table_partition_noalloc": (C.c_int, [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P, U64P,
+                                             U64P]),
     "wk_table_final_process": (C.c_int, [P, P, I32P, C.c_int32, I32P, C.c_uint32, C.c_int32, I32P, I32P,
                                          C.c_uint32, C.c_int64, C.c_int64, C.POINTER(P)]),
     "wk_dev_sync": (C.c_int, [P]),
