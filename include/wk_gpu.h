@@ -272,6 +272,13 @@ int wk_expand_k2k(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t 
 int wk_expand_k2c(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t pid, int32_t dir,
                   uint32_t end, wk_table** out);
 
+/* const_to_known: keep rows whose row[end_col] is in the edges of
+ * [vid|pid|dir] (core/engine/sparql.hpp:139-188); with vid = 0 the edges are
+ * the index list [0|pid|dir], the index_to_known of a request that arrives
+ * with a table (sparql.hpp:83-137).  Each row is kept at most once. */
+int wk_expand_c2k(wk_dev* dev, const wk_table* in, uint32_t vid, uint32_t pid, int32_t dir, uint32_t end_col,
+                  wk_table** out);
+
 /* ------------------------------------------------------------------ query */
 
 /* Run one query pattern by pattern (per-query GPU execution, one host sync

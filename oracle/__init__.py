@@ -44,6 +44,16 @@ def lib():
         L.orc_run_query.restype = C.c_int
         L.orc_run_query.argtypes = [C.c_void_p, I32P, C.c_uint32, C.c_int32, C.POINTER(U32P), U64P,
                                     C.POINTER(C.c_uint32), I32P]
+        L.orc_run_query_from.restype = C.c_int
+        L.orc_run_query_from.argtypes = [C.c_void_p, I32P, C.c_uint32, C.c_int32, C.c_uint32, U32P, C.c_uint64,
+                                         C.c_uint32, I32P, C.POINTER(U32P), U64P, C.POINTER(C.c_uint32), I32P]
+        L.orc_run_query_mt.restype = C.c_int
+        L.orc_run_query_mt.argtypes = [C.c_void_p, I32P, C.c_uint32, C.c_int32, C.c_int, C.POINTER(U32P), U64P,
+                                       C.POINTER(C.c_uint32), I32P]
+        L.orc_run_stream.restype = C.c_int
+        L.orc_run_stream.argtypes = [C.c_void_p, I32P, U64P, I32P, C.c_uint64, C.c_int, U64P, U64P, I32P]
+        L.orc_table_checksum.restype = C.c_uint64
+        L.orc_table_checksum.argtypes = [U32P, C.c_uint64, C.c_uint32]
         L.orc_set_direction.argtypes = [I32P, I32P, I32P, C.c_uint32, I32P]
         L.orc_final_process.restype = C.c_int
         L.orc_final_process.argtypes = [U32P, C.c_uint64, C.c_uint32, I32P, C.c_int32, I32P, C.c_uint32, C.c_int,
@@ -111,6 +121,47 @@ class Store:
         v2c = np.zeros(max(1, nvars), np.int32)
         rc = lib().orc_run_query(self._h, pats.ctypes.data_as(I32P), len(patterns), nvars, C.byref(tp),
                                  C.byref(rows), C.byref(cols), v2c.ctypes.data_as(I32P))
+        return self._take(rc, tp, rows, cols, v2c, nvars)
+
+    def query_from(self, patterns, nvars, step, table, v2c_in):
+        """Patterns [step, n) over a carried table with var->column map v2c_in."""
+        pats = np.asarray(patterns, dtype=np.int32).reshape(-1)
+        t = np.ascontiguousarray(table, dtype=np.uint32)
+        vin = np.full(max(1, nvars), -1, np.int32)
+        vin[:len(v2c_in)] = v2c_in
+        tp, rows, cols = U32P(), C.c_uint64(), C.c_uint32()
+        v2c = np.zeros(max(1, nvars), np.int32)
+        rc = lib().orc_run_query_from(self._h, pats.ctypes.data_as(I32P), len(patterns), nvars, step,
+                                      t.ctypes.data_as(U32P), t.shape[0], t.shape[1], vin.ctypes.data_as(I32P),
+                                      C.byref(tp), C.byref(rows), C.byref(cols), v2c.ctypes.data_as(I32P))
+        return self._take(rc, tp, rows, cols, v2c, nvars)
+
+    def query_mt(self, patterns, nvars, nthreads):
+        """A query from an index split over nthreads threads (mt_factor)."""
+        pats = np.asarray(patterns, dtype=np.int32).reshape(-1)
+        tp, rows, cols = U32P(), C.c_uint64(), C.c_uint32()
+        v2c = np.zeros(max(1, nvars), np.int32)
+        rc = lib().orc_run_query_mt(self._h, pats.ctypes.data_as(I32P), len(patterns), nvars, nthreads,
+                                    C.byref(tp), C.byref(rows), C.byref(cols), v2c.ctypes.data_as(I32P))
+        return self._take(rc, tp, rows, cols, v2c, nvars)
+
+    def run_stream(self, queries, nthreads):
+        """[(patterns, nvars)] on nthreads workers sharing one queue ->
+        (rows, checksums, status) arrays."""
+        nq = len(queries)
+        off = np.zeros(nq + 1, np.uint64)
+        off[1:] = np.cumsum([len(p) for p, _ in queries])
+        pats = np.asarray([x for p, _ in queries for pat in p for x in pat] or [0], dtype=np.int32)
+        nv = np.asarray([n for _, n in queries] or [0], dtype=np.int32)
+        rows = np.zeros(max(1, nq), np.uint64)
+        ck = np.zeros(max(1, nq), np.uint64)
+        st = np.zeros(max(1, nq), np.int32)
+        lib().orc_run_stream(self._h, pats.ctypes.data_as(I32P), off.ctypes.data_as(U64P), nv.ctypes.data_as(I32P),
+                             nq, nthreads, rows.ctypes.data_as(U64P), ck.ctypes.data_as(U64P),
+                             st.ctypes.data_as(I32P))
+        return rows[:nq], ck[:nq], st[:nq]
+
+    def _take(self, rc, tp, rows, cols, v2c, nvars):
         if rc:
             return np.zeros((0, 0), np.uint32), [], rc
         n = rows.value * cols.value
@@ -119,6 +170,14 @@ class Store:
         if tp:
             lib().orc_free(C.cast(tp, C.c_void_p))
         return t, [int(x) for x in v2c[:nvars]], 0
+
+
+def table_checksum(table):
+    """Order-independent fingerprint of a table (sum of per-row hashes)."""
+    t = np.ascontiguousarray(table, dtype=np.uint32)
+    if t.ndim == 1:
+        t = t.reshape(-1, 1)
+    return int(lib().orc_table_checksum(t.ctypes.data_as(U32P), t.shape[0], t.shape[1]))
 
 
 def final_process(table, v2c, required, distinct=False, orders=(), offset=0, limit=-1):

@@ -420,21 +420,20 @@ static int member(const uint32_t* e, uint64_t n, uint32_t x) {
 
 enum { KNOWN_VAR = 0, UNKNOWN_VAR = 1, CONST_VAR = 2 };
 
-int orc_run_query(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, uint32_t** table,
-                  uint64_t* rows, uint32_t* cols, int32_t* v2c) {
-  vec res = {0, 0, 0}, upd = {0, 0, 0};
-  uint32_t col_num = 0, step;
-  int rc = 0, i;
+/* Steps [from, npat) of a planned pattern list over the history table *res
+ * of *col_io columns (empty for a query from its first pattern) with the
+ * var->column map v2c, as execute_patterns loops execute_one_pattern
+ * (sparql.hpp:932-1055,1107-1148).  *res holds the result on success. */
+static int exec_steps(const orc_store* st, const int32_t* pats, uint32_t from, uint32_t npat, int32_t nvars,
+                      vec* res_io, uint32_t* col_io, int32_t* v2c) {
+  vec res = *res_io, upd = {0, 0, 0};
+  uint32_t col_num = *col_io, step;
+  int rc = 0;
   scratch sc = {0, 0};
-  *table = NULL;
-  *rows = 0;
-  *cols = 0;
-  if (nvars < 0 || nvars > 64) return E_TOO_MANY_VARS;
-  for (i = 0; i < nvars; i++) v2c[i] = -1;
 #define STAT(x) ((x) >= 0 ? CONST_VAR : (v2c[-((x) + 1)] < 0 ? UNKNOWN_VAR : KNOWN_VAR))
 #define COL(x) ((uint32_t)v2c[-((x) + 1)])
 #define BAD_VAR(x) ((x) < 0 && -((int64_t)(x) + 1) >= nvars)
-  for (step = 0; step < npat && rc == 0; step++) {
+  for (step = from; step < npat && rc == 0; step++) {
     const int32_t s = pats[4 * step], p = pats[4 * step + 1], o = pats[4 * step + 2], d = pats[4 * step + 3];
     const uint64_t nrows = col_num ? res.n / col_num : 0;
     uint64_t r, sz, k;
@@ -449,7 +448,7 @@ int orc_run_query(const orc_store* st, const int32_t* pats, uint32_t npat, int32
     }
     upd.n = 0;
     if (step == 0 && s >= 0 && is_tpid((uint32_t)s)) {
-      /* index_to_unknown */
+      /* the first pattern from an index (start_from_index, query.hpp:816-838) */
       if (p != (int32_t)PREDICATE_ID && p != (int32_t)TYPE_ID) {
         rc = E_OBJ_ERROR;
         break;
@@ -459,9 +458,22 @@ int orc_run_query(const orc_store* st, const int32_t* pats, uint32_t npat, int32
         break;
       }
       e = index_list(st, (uint32_t)s, d, &sz);
-      for (k = 0; k < sz && !rc; k++) rc = vpush(&upd, e[k]);
-      v2c[-(o + 1)] = 0;
-      col_num = 1;
+      if (v2c[-(o + 1)] >= 0) {
+        /* index_to_known (sparql.hpp:83-137): keep the rows of the carried
+         * table whose ?o is in the index list, each once */
+        const uint32_t c = COL(o);
+        for (r = 0; r < nrows && !rc; r++)
+          if (member(e, sz, res.d[r * col_num + c])) rc = append_row(&upd, res.d + r * col_num, col_num);
+      } else {
+        /* index_to_unknown (sparql.hpp:197-234) */
+        if (col_num != 0) {
+          rc = E_FIRST_PATTERN;
+          break;
+        }
+        for (k = 0; k < sz && !rc; k++) rc = vpush(&upd, e[k]);
+        v2c[-(o + 1)] = 0;
+        col_num = 1;
+      }
     } else if (p < 0 || p >= (int32_t)TPID_LIMIT) {
       rc = E_UNKNOWN_PATTERN;
       break;
@@ -526,14 +538,156 @@ int orc_run_query(const orc_store* st, const int32_t* pats, uint32_t npat, int32
 #undef BAD_VAR
   free(upd.d);
   free(sc.buf);
+  *res_io = res;
+  *col_io = col_num;
+  return rc;
+}
+
+static int finish_query(int rc, vec* res, uint32_t col_num, int32_t nvars, uint32_t** table, uint64_t* rows,
+                        uint32_t* cols, int32_t* v2c) {
+  int i;
   if (rc) {
-    free(res.d);
+    free(res->d);
     for (i = 0; i < nvars; i++) v2c[i] = -1;
     return rc;
   }
-  *table = res.d;
-  *rows = col_num ? res.n / col_num : 0;
+  *table = res->d;
+  *rows = col_num ? res->n / col_num : 0;
   *cols = col_num;
+  return 0;
+}
+
+int orc_run_query(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, uint32_t** table,
+                  uint64_t* rows, uint32_t* cols, int32_t* v2c) {
+  vec res = {0, 0, 0};
+  uint32_t col_num = 0;
+  int i;
+  *table = NULL;
+  *rows = 0;
+  *cols = 0;
+  if (nvars < 0 || nvars > 64) return E_TOO_MANY_VARS;
+  for (i = 0; i < nvars; i++) v2c[i] = -1;
+  const int rc = exec_steps(st, pats, 0, npat, nvars, &res, &col_num, v2c);
+  return finish_query(rc, &res, col_num, nvars, table, rows, cols, v2c);
+}
+
+int orc_run_query_from(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, uint32_t from,
+                       const uint32_t* in, uint64_t in_rows, uint32_t in_cols, const int32_t* in_v2c,
+                       uint32_t** table, uint64_t* rows, uint32_t* cols, int32_t* v2c) {
+  vec res = {0, 0, 0};
+  uint64_t k;
+  int i;
+  *table = NULL;
+  *rows = 0;
+  *cols = 0;
+  if (nvars < 0 || nvars > 64) return E_TOO_MANY_VARS;
+  for (i = 0; i < nvars; i++) v2c[i] = in_v2c[i];
+  for (k = 0; k < in_rows * in_cols; k++)
+    if (vpush(&res, in[k])) {
+      free(res.d);
+      return E_NOMEM;
+    }
+  {
+    const int rc = exec_steps(st, pats, from, npat, nvars, &res, &in_cols, v2c);
+    return finish_query(rc, &res, in_cols, nvars, table, rows, cols, v2c);
+  }
+}
+
+int orc_run_query_mt(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, int nthreads,
+                     uint32_t** table, uint64_t* rows, uint32_t* cols, int32_t* v2c) {
+  uint64_t sz = 0, total = 0;
+  vec* part;
+  int32_t* pv;
+  uint32_t ncol = 0;
+  int t, rc = 0, i;
+  const uint32_t* e;
+  /* only a query from an index is split (dispatch, sparql.hpp:1058-1083) */
+  if (nthreads <= 1 || npat == 0 || pats[0] < 0 || !is_tpid((uint32_t)pats[0]) || pats[2] >= 0 ||
+      (pats[1] != (int32_t)PREDICATE_ID && pats[1] != (int32_t)TYPE_ID) || nvars < 1 || nvars > 64 ||
+      -((int64_t)pats[2] + 1) >= nvars || (pats[3] != DIR_IN && pats[3] != DIR_OUT))
+    return orc_run_query(st, pats, npat, nvars, table, rows, cols, v2c);
+  *table = NULL;
+  *rows = 0;
+  *cols = 0;
+  e = index_list(st, (uint32_t)pats[0], pats[3], &sz);
+  part = (vec*)calloc((size_t)nthreads, sizeof(vec));
+  pv = (int32_t*)malloc((size_t)nthreads * 64 * sizeof(int32_t));
+  if (!part || !pv) {
+    free(part);
+    free(pv);
+    return E_NOMEM;
+  }
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1) reduction(| : rc)
+  for (t = 0; t < nthreads; t++) {
+    /* index_to_unknown with mt_factor = nthreads, mt_tid = t (sparql.hpp:214-224):
+     * consecutive parts of length sz / nthreads, the rest to the last */
+    const uint64_t len = sz / (uint64_t)nthreads;
+    const uint64_t lo = (uint64_t)t * len, hi = (t == nthreads - 1) ? sz : lo + len;
+    uint32_t c = 1;
+    uint64_t k;
+    int j, r = 0;
+    int32_t* v = pv + (size_t)t * 64;
+    for (j = 0; j < nvars; j++) v[j] = -1;
+    v[-(pats[2] + 1)] = 0;
+    for (k = lo; k < hi && !r; k++) r = vpush(&part[t], e[k]);
+    if (!r) r = exec_steps(st, pats, 1, npat, nvars, &part[t], &c, v);
+    if (t == 0) ncol = c;
+    rc |= r;
+  }
+  for (i = 0; i < nvars; i++) v2c[i] = pv[i];
+  if (!rc) {
+    vec all = {0, 0, 0};
+    for (t = 0; t < nthreads && !rc; t++) total += part[t].n;
+    all.d = (uint32_t*)malloc((total ? total : 1) * sizeof(uint32_t));
+    if (!all.d) rc = E_NOMEM;
+    for (t = 0; t < nthreads && !rc; t++) {
+      memcpy(all.d + all.n, part[t].d, part[t].n * sizeof(uint32_t));
+      all.n += part[t].n;
+    }
+    for (t = 0; t < nthreads; t++) free(part[t].d);
+    free(part);
+    free(pv);
+    return finish_query(rc, &all, ncol, nvars, table, rows, cols, v2c);
+  }
+  for (t = 0; t < nthreads; t++) free(part[t].d);
+  free(part);
+  free(pv);
+  for (i = 0; i < nvars; i++) v2c[i] = -1;
+  return rc;
+}
+
+uint64_t orc_table_checksum(const uint32_t* table, uint64_t rows, uint32_t cols) {
+  /* sum over rows of a chained hash of (column, value) pairs: the same
+   * order-independent fingerprint as the device library's wk_table_checksum */
+  uint64_t s = 0, r;
+  uint32_t c;
+  for (r = 0; r < rows; r++) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ cols;
+    for (c = 0; c < cols; c++) h = orc_hash_u64(h ^ ((uint64_t)table[r * cols + c] + ((uint64_t)c << 32)));
+    s += h;
+  }
+  return s;
+}
+
+int orc_run_stream(const orc_store* st, const int32_t* pats, const uint64_t* pat_off, const int32_t* nvars,
+                   uint64_t nq, int nthreads, uint64_t* rows, uint64_t* checksums, int32_t* status) {
+  int64_t i;
+  if (nthreads < 1) nthreads = 1;
+  /* one worker per thread pulling the next query from a shared queue, as the
+   * engine threads of Engine::run take requests (engine.hpp:132-233) */
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+  for (i = 0; i < (int64_t)nq; i++) {
+    uint32_t* t = NULL;
+    uint64_t r = 0;
+    uint32_t c = 0;
+    int32_t v[64];
+    const int rc = orc_run_query(st, pats + 4 * pat_off[i], (uint32_t)(pat_off[i + 1] - pat_off[i]), nvars[i], &t,
+                                 &r, &c, v);
+    status[i] = rc;
+    rows[i] = rc ? 0 : r;
+    checksums[i] = rc ? 0 : orc_table_checksum(t, r, c);
+    free(t);
+  }
   return 0;
 }
 

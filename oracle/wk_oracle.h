@@ -46,6 +46,30 @@ const uint32_t* orc_get_index(const orc_store* st, uint32_t tpid, int dir, uint6
 int orc_run_query(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, uint32_t** table,
                   uint64_t* rows, uint32_t* cols, int32_t* v2c);
 
+/* The same, from pattern `from` over a carried history table (in_rows x
+ * in_cols, var->column map in_v2c[nvars]), as a request that arrives with a
+ * table continues (GPUEngine::load_result_buf, gpu/gpu_engine.hpp:248-263);
+ * an index pattern at step 0 whose object is bound is index_to_known. */
+int orc_run_query_from(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, uint32_t from,
+                       const uint32_t* in, uint64_t in_rows, uint32_t in_cols, const int32_t* in_v2c,
+                       uint32_t** table, uint64_t* rows, uint32_t* cols, int32_t* v2c);
+
+/* orc_run_query with a query from an index split over nthreads threads the
+ * way mt_factor splits it (sparql.hpp:214-224): thread t takes consecutive
+ * part t of the index list (the last also the remainder) and runs the
+ * remaining patterns on it; the parts' tables are concatenated in thread
+ * order.  Other queries run as orc_run_query. */
+int orc_run_query_mt(const orc_store* st, const int32_t* pats, uint32_t npat, int32_t nvars, int nthreads,
+                     uint32_t** table, uint64_t* rows, uint32_t* cols, int32_t* v2c);
+
+/* nq queries (query i: patterns [pat_off[i], pat_off[i+1]) of `pats`, nvars[i]
+ * variables) on nthreads workers pulling from one shared queue (Engine::run,
+ * core/engine/engine.hpp:132-233): row count, order-independent checksum
+ * (orc_table_checksum) and status of each. */
+int orc_run_stream(const orc_store* st, const int32_t* pats, const uint64_t* pat_off, const int32_t* nvars,
+                   uint64_t nq, int nthreads, uint64_t* rows, uint64_t* checksums, int32_t* status);
+uint64_t orc_table_checksum(const uint32_t* table, uint64_t rows, uint32_t cols);
+
 /* Planner::set_direction (optimizer/planner.hpp:667-699): rewrite raw
  * patterns by a .fmt plan; dirs: 0 "<", 1 ">", 2 "<<", 3 ">>". */
 void orc_set_direction(const int32_t* raw, const int32_t* orders, const int32_t* dirs, uint32_t n,

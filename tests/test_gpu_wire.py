@@ -109,3 +109,60 @@ def test_agent_status_codes(univ10_dev):
     assert e.value.code == wg.WK_E_WIRE
     # the failed call leaves no reply behind
     assert wg.lib().wk_agent_reply(univ10_dev._h, None, 0) == wg.WK_E_STATE
+
+
+def test_agent_index_to_known_on_a_carried_table(univ10_dev, univ10_oracle):
+    """A request at pattern_step 0 that carries a history table, whose first
+    pattern is an index pattern with its object bound (index_to_known,
+    sparql.hpp:83-137): the device keeps the rows whose value is in the index
+    list, each once, then goes on; replies equal the oracle's continuation of
+    the same table.  A constant start with a bound object after the table is
+    const_to_known (sparql.hpp:139-188)."""
+    GS, US, MEM, SUB = (Q.U("ub:GraduateStudent"), Q.U("ub:UndergraduateStudent"), Q.U("ub:memberOf"),
+                        Q.U("ub:subOrganizationOf"))
+    g, _, _ = univ10_oracle.query([(GS, wg.TYPE_ID, -1, wg.IN), (-1, MEM, -2, wg.OUT)], 2)
+    u, _, _ = univ10_oracle.query([(US, wg.TYPE_ID, -1, wg.IN), (-1, MEM, -2, wg.OUT)], 2)
+    table = np.concatenate([u[:3000], g[:2000], u[:500]])  # duplicate rows stay duplicates
+    dept = int(g[0, 1])
+    cases = [
+        [(GS, wg.TYPE_ID, -1, wg.IN), (-2, SUB, -3, wg.OUT)],      # type index, then k2u
+        [(MEM, wg.PREDICATE_ID, -1, wg.IN)],                         # predicate index [0|p|IN]
+        [(US, wg.TYPE_ID, -1, wg.IN), (dept, MEM, -1, wg.IN)],       # then const_to_known
+    ]
+    for pats in cases:
+        want, wv, wrc = univ10_oracle.query_from(pats, 3, 0, table, [0, 1])
+        assert wrc == 0 and want.shape[0] > 0, pats
+        req = bound_vars(wv)
+        m = wg.query_msg(pats, 3, req, pattern_step=0, col_num=2, row_num=table.shape[0],
+                         table=table.reshape(-1), v2c_map=[0, 1, NO_RESULT])
+        r = run(univ10_dev, m)
+        assert r["status_code"] == 0, pats
+        expect = oracle.final_process(want, wv, req)
+        got = r["table"].reshape(-1, len(req))
+        assert r["row_num"] == expect.shape[0] and same_rows(got, expect), pats
+    # an unbound object after a table, and a constant object, are refused
+    for pats, code in [([(GS, wg.TYPE_ID, -3, wg.IN)], wg.WK_E_FIRST_PATTERN),
+                       ([(GS, wg.TYPE_ID, dept, wg.IN)], wg.WK_E_VERTEX_INVALID),
+                       ([(GS, MEM, -1, wg.IN)], wg.WK_E_OBJ_ERROR)]:
+        m = wg.query_msg(pats, 3, [-1], pattern_step=0, col_num=2, row_num=table.shape[0],
+                         table=table.reshape(-1), v2c_map=[0, 1, NO_RESULT])
+        assert run(univ10_dev, m)["status_code"] == code, pats
+        assert univ10_oracle.query_from(pats, 3, 0, table, [0, 1])[2] == code, pats
+
+
+def test_c2k_entry_point(univ10_dev, univ10_oracle):
+    """wk_expand_c2k on an uploaded table: index lists (vid 0) and a vertex's
+    edges, both directions."""
+    GS, MEM = Q.U("ub:GraduateStudent"), Q.U("ub:memberOf")
+    people = np.sort(np.concatenate([univ10_oracle.get_index(GS, wg.IN),
+                                     univ10_oracle.get_index(Q.U("ub:UndergraduateStudent"), wg.IN)]))
+    t = univ10_dev.upload(people.reshape(-1, 1))
+    dept = int(univ10_oracle.get_triples(int(people[0]), MEM, wg.OUT)[0])
+    for vid, pid, d in [(0, GS, wg.IN), (0, MEM, wg.IN), (dept, MEM, wg.IN)]:
+        out = univ10_dev.c2k(t, vid, pid, d, 0)
+        members = set((univ10_oracle.get_index(pid, d) if vid == 0 else
+                       univ10_oracle.get_triples(vid, pid, d)).tolist())
+        want = people[np.array([int(v) in members for v in people], bool)].reshape(-1, 1)
+        assert want.shape[0] > 0 and same_rows(out.numpy(), want), (vid, pid, d)
+        out.free()
+    t.free()

@@ -1,7 +1,9 @@
 """Pinning the CPU oracle (CPU only).
 
-* hash: the forward mix and the independently written inverse
-  (utils/math.hpp:59-104) must compose to the identity;
+* hash: the forward mix composed with the reference's own inverse, restated
+  step for step from utils/math.hpp:70-104, must be the identity (this pins
+  the forward mix against the reference's inverse, not against a second
+  derivation);
 * store: the reference's own store fixture (tests/test_graph.cc:142-226),
   restated with its asserted sizes and values;
 * engine: brute-force BGP evaluation (all variable assignments over the
@@ -135,6 +137,13 @@ def tiny_query_set(info):
         ([(v0, p0, -1, OUT), (-1, p1, -2, OUT), (-2, p0, -3, IN)], 3),
         ([(p0, 0, -1, IN), (-1, p0, -2, OUT), (-2, p0, -1, OUT)], 2),
         ([(p1, 0, -1, OUT), (-1, p1, -2, IN), (v0, p2, -2, OUT)], 2),
+        # known_to_unknown with rdf:type IN: instances of ?t (get_index(cur, IN), sparql.hpp:342-343)
+        ([(v0, TYPE_ID, -1, OUT), (-1, TYPE_ID, -2, IN)], 2),
+        ([(t0, TYPE_ID, -1, IN), (-1, TYPE_ID, -2, OUT), (-2, TYPE_ID, -3, IN)], 3),
+        # const_to_known, both directions (sparql.hpp:139-188)
+        ([(t0, TYPE_ID, -1, IN), (v0, p0, -1, OUT)], 1),
+        ([(p0, 0, -1, IN), (v0, p1, -1, IN)], 1),
+        ([(t1, TYPE_ID, -1, IN), (-1, p0, -2, OUT), (v0, p0, -2, IN)], 2),
     ]
 
 
@@ -150,6 +159,99 @@ def test_oracle_matches_brute_force(seed):
         cols = [c for c in cols if v2c[c] >= 0]
         want = np.array([[m[-(c + 1)] for c in cols] for m in sols], np.uint32).reshape(len(sols), len(cols))
         assert same_rows(t, want), pats
+
+
+def _triple_set(tri):
+    return set(map(tuple, tri.tolist()))
+
+
+def _index_members(T, s, p, d):
+    """The index list [0|s|d] from the triple set itself."""
+    if p == TYPE_ID:
+        return {a for (a, b, c) in T if b == TYPE_ID and c == s}
+    if d == IN:
+        return {a for (a, b, c) in T if b == s}
+    return {c for (a, b, c) in T if b == s and c >= VID0}
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_index_to_known_on_a_carried_table(seed):
+    """A request arriving with a history table whose first pattern is an index
+    pattern with a bound object (index_to_known, sparql.hpp:83-137): each row
+    is kept once when its value is in the index list, computed here from the
+    triple set; then the query goes on from the filtered table."""
+    tri, info = random_graph(20 + seed, n_v=50, n_edges=300)
+    st = oracle.Store(tri)
+    T = _triple_set(tri)
+    rng = np.random.default_rng(seed)
+    verts = info["verts"]
+    table = np.stack([rng.choice(verts, 80), rng.choice(verts, 80)], axis=1).astype(np.uint32)
+    table = np.concatenate([table, table[:5]])          # duplicate rows stay duplicates
+    t0, p0 = int(info["types"][0]), int(info["preds"][0])
+    for pat, col in [((t0, TYPE_ID, -1, IN), 0), ((p0, 0, -2, IN), 1), ((p0, 0, -1, OUT), 0)]:
+        got, v2c, rc = st.query_from([pat], 2, 0, table, [0, 1])
+        assert rc == 0 and v2c == [0, 1]
+        members = _index_members(T, pat[0], pat[1], pat[3])
+        want = table[np.array([int(v) in members for v in table[:, col]], bool)]
+        assert np.array_equal(got, want), pat
+    # then on to a known_to_unknown step
+    got, v2c, rc = st.query_from([(t0, TYPE_ID, -1, IN), (-1, p0, -3, OUT)], 3, 0, table, [0, 1])
+    kept = table[np.array([int(v) in _index_members(T, t0, TYPE_ID, IN) for v in table[:, 0]], bool)]
+    want = [(a, b, c) for a, b in kept.tolist() for (x, y, c) in sorted(T) if x == a and y == p0]
+    assert rc == 0 and v2c == [0, 1, 2]
+    assert same_rows(got, np.array(want, np.uint32).reshape(-1, 3))
+    # status codes: unbound object after a table, constant object, bad predicate
+    assert st.query_from([(t0, TYPE_ID, -3, IN)], 3, 0, table, [0, 1])[2] == 11
+    assert st.query_from([(t0, TYPE_ID, VID0, IN)], 2, 0, table, [0, 1])[2] == 8
+    assert st.query_from([(t0, p0, -1, IN)], 2, 0, table, [0, 1])[2] == 7
+
+
+def test_query_from_continues_a_query():
+    """Running patterns [0, k) then [k, n) from the intermediate table equals
+    running all n."""
+    tri, info = random_graph(31, n_v=60, n_edges=500)
+    st = oracle.Store(tri)
+    for pats, nv in tiny_query_set(info):
+        full, v2c, rc = st.query(pats, nv)
+        for k in range(1, len(pats)):
+            part, pv, prc = st.query(pats[:k], nv)
+            assert prc == 0
+            got, gv, grc = st.query_from(pats, nv, k, part if part.size else part.reshape(0, max(1, len(
+                [x for x in pv if x >= 0]))), pv)
+            assert grc == rc and gv == v2c and np.array_equal(got, full), (pats, k)
+
+
+@pytest.mark.parametrize("nthreads", [1, 2, 3, 7, 64])
+def test_mt_split_equals_one_thread(nthreads):
+    """mt_factor split of index starts (sparql.hpp:214-224) over up to more
+    threads than index entries: the parts' tables in thread order are the
+    one-thread table row for row (each part keeps the list's order)."""
+    tri, info = random_graph(41, n_v=120, n_edges=900)
+    st = oracle.Store(tri)
+    for pats, nv in tiny_query_set(info):
+        want, wv, wrc = st.query(pats, nv)
+        got, gv, grc = st.query_mt(pats, nv, nthreads)
+        assert grc == wrc and gv == wv
+        assert np.array_equal(got, want), pats
+
+
+def test_run_stream_matches_queries():
+    """The shared-queue stream runner gives each query's row count, status and
+    fingerprint as the query run alone does, on 1 and 8 workers."""
+    import wukong_gpu as wg
+    from _graphs import random_query
+    tri, info = random_graph(51, n_v=200, n_edges=2000)
+    st = oracle.Store(tri)
+    rng = np.random.default_rng(2)
+    qs = [random_query(rng, info) for _ in range(300)]
+    for nt in (1, 8):
+        rows, ck, status = st.run_stream(qs, nt)
+        for i, (pats, nv) in enumerate(qs):
+            t, _, rc = st.query(pats, nv)
+            assert status[i] == rc
+            if rc == 0:
+                assert rows[i] == t.shape[0]
+                assert int(ck[i]) == oracle.table_checksum(t) == (wg.checksum(t) if t.size else 0)
 
 
 def test_oracle_matches_sqlite():
@@ -249,6 +351,32 @@ def test_final_process_projection_distinct_order_limit():
     # empty table stays untouched
     e = np.zeros((0, 3), np.uint32)
     assert oracle.final_process(e, v2c, [-1]).shape == (0, 3)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_distinct_properties(seed):
+    """DISTINCT by properties.  The reference sorts the rows on all columns
+    and drops a row when it equals the last kept row on the required columns
+    (sparql.hpp:1441-1465), so: every projected tuple survives at least once,
+    no two consecutive kept rows are equal on the required columns, and when
+    the required variables cover every column the result is exactly the set
+    of rows.  (With fewer required columns equal tuples need not be adjacent
+    after the full-row sort, and the reference keeps each run of them.)"""
+    rng = np.random.default_rng(seed)
+    t = rng.integers(0, 4, size=(200, 4)).astype(np.uint32)
+    t[:20, 1] |= np.uint32(1 << 31)  # words above 2^31 (signed order in the sort)
+    v2c = [3, 1, 0, 2]
+    for req in ([-1], [-2, -4], [-4, -1, -3], [-1, -2, -3, -4], [-3, -4, -2, -1]):
+        out = oracle.final_process(t, v2c, req, distinct=True)
+        proj = {tuple(r) for r in t[:, [v2c[-(v + 1)] for v in req]].tolist()}
+        got = [tuple(r) for r in out.tolist()]
+        assert set(got) == proj and len(got) <= len(t)
+        assert all(a != b for a, b in zip(got, got[1:]))
+        if len(req) == t.shape[1]:
+            assert len(got) == len(proj)
+    # the same rows in the order of the required columns: a true set
+    out = oracle.final_process(t, [0, 1, 2, 3], [-1, -2, -3, -4], distinct=True)
+    assert len(out) == len({tuple(r) for r in t.tolist()})
 
 
 def test_partition_rows_matches_the_reference_loop():

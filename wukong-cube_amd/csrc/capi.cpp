@@ -577,6 +577,25 @@ int wk_expand_k2c(wk_dev* dev, const wk_table* in, uint32_t start_col, uint32_t 
   return known_step(dev, in, start_col, pid, dir, OP_FILTER_CONST, end, out);
 }
 
+int wk_expand_c2k(wk_dev* dev, const wk_table* in, uint32_t vid, uint32_t pid, int32_t dir, uint32_t end_col,
+                  wk_table** out) {
+  if (!dev || !in || !out || (dir != WK_IN && dir != WK_OUT) || pid >= (1u << NBITS_IDX)) return WK_E_INVALID_ARG;
+  if (end_col >= in->cols) return WK_E_INVALID_ARG;
+  StepDesc sd{};
+  sd.op = OP_FILTER_KNOWN;
+  sd.key_mode = KEY_CONST;
+  sd.kstart = vid;
+  sd.pid = pid;
+  sd.dir = uint32_t(dir);
+  sd.ec = end_col;
+  sd.in_cols = in->cols;
+  sd.out_cols = in->cols;
+  sd.in_rows = in->rows;
+  const SegId s = segid_of_key(make_key(vid, pid, uint64_t(dir)));
+  sd.seg = dev->st->seg_index(s.index, s.pid, s.dir);
+  return pattern_step(dev, sd, in, out);
+}
+
 // ------------------------------------------------------------------ query
 
 int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols, int32_t* v2c) {

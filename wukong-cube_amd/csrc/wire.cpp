@@ -314,11 +314,31 @@ int continue_patterns(wk_dev* dev, const wk_sparql_msg& m, std::vector<int32_t>&
     auto idx = [&](int32_t x) { return -(int64_t(x) + 1); };
     for (int32_t x : {p.subject, p.object})
       if (x < 0 && idx(x) >= m.nvars) return WK_E_VERTEX_INVALID;
-    // a constant, index or unbound start cannot follow a history table
-    if (p.subject >= 0 || v2c[size_t(idx(p.subject))] < 0) return WK_E_FIRST_PATTERN;
-    const uint32_t sc = uint32_t(v2c[size_t(idx(p.subject))]);
     wk_table* next = nullptr;
     int rc;
+    if (p.subject >= 0) {
+      // a constant start over the carried table: the first pattern from an
+      // index (start_from_index, query.hpp:816-838) is index_to_known when
+      // its object is bound (sparql.hpp:83-137, dispatched at :945-950);
+      // otherwise const_to_known (:139-188).  An unbound object would start
+      // a new table, which must be the first pattern.
+      const bool index_start = k == 0 && is_tpid(p.subject);
+      if (index_start && p.predicate != int32_t(PREDICATE_ID) && p.predicate != int32_t(TYPE_ID))
+        return WK_E_OBJ_ERROR;
+      if (p.object >= 0) return index_start ? WK_E_VERTEX_INVALID : WK_E_UNKNOWN_PATTERN;
+      const int32_t ec = v2c[size_t(idx(p.object))];
+      if (ec < 0) return WK_E_FIRST_PATTERN;
+      rc = index_start ? wk_expand_c2k(dev, *t, 0, uint32_t(p.subject), p.direction, uint32_t(ec), &next)
+                       : wk_expand_c2k(dev, *t, uint32_t(p.subject), uint32_t(p.predicate), p.direction,
+                                       uint32_t(ec), &next);
+      if (rc) return rc;
+      (void)wk_table_free(dev, *t);
+      *t = next;
+      continue;
+    }
+    // an unbound start cannot follow a history table
+    if (v2c[size_t(idx(p.subject))] < 0) return WK_E_FIRST_PATTERN;
+    const uint32_t sc = uint32_t(v2c[size_t(idx(p.subject))]);
     if (p.object >= 0) {
       rc = wk_expand_k2c(dev, *t, sc, uint32_t(p.predicate), p.direction, uint32_t(p.object), &next);
     } else if (v2c[size_t(idx(p.object))] >= 0) {

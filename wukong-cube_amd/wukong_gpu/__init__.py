@@ -175,6 +175,7 @@ SIGNATURES = {
     "wk_expand_k2u": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(P)]),
     "wk_expand_k2k": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, C.POINTER(P)]),
     "wk_expand_k2c": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, C.POINTER(P)]),
+    "wk_expand_c2k": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, C.POINTER(P)]),
     "wk_query_run": (C.c_int, [P, C.POINTER(Query), U64P, U32P, C.POINTER(C.c_int32)]),
     "wk_result_download": (C.c_int, [P, U32P, C.c_uint64]),
     "wk_batch_run": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(BatchResult)]),
@@ -631,6 +632,11 @@ class Device:
     def k2c(self, t, col, pid, d, end):
         h = P()
         _check(lib().wk_expand_k2c(self._h, t._h, col, pid, d, end, C.byref(h)), "wk_expand_k2c")
+        return Table(self, h)
+
+    def c2k(self, t, vid, pid, d, end_col):
+        h = P()
+        _check(lib().wk_expand_c2k(self._h, t._h, vid, pid, d, end_col, C.byref(h)), "wk_expand_c2k")
         return Table(self, h)
 
     # whole queries
