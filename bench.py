@@ -2,26 +2,33 @@
 
 Default workload: the university-schema graph at 2,560 universities (seed 1,
 ~339M triples, gstore resident in HBM):
-  * single-query kernels: Q1 = the lubm_q1 shape (6 patterns with its OSDI16
-    plan, a known_to_known join), Q2 = the 2-pattern type-start query.
-    Per-kernel HIP-event timing gives the roofline of the dominant kernel;
-  * one "step" = one batched pass over a stream of 4,096 selective queries
-    from 5 templates (constants drawn from the type index).  Each pattern step
-    of all queries is one set of launches over a concatenated table.  The
-    native autotuner first picks the probe width and the gather mapping;
+  * single-query kernels: Q1 = lubm_q7 (6-pattern cyclic query with its
+    OSDI16 plan and a known_to_known closing edge), Q2 = the 2-pattern
+    type-start query, and lubm_q1.  Per-kernel HIP-event timing gives the
+    roofline of the dominant kernel;
+  * one "step" = one batched pass over a distinct slice of 4,096 selective
+    queries of a stream from 5 templates (constants drawn from the type
+    index), so no timed step replays an earlier batch.  Each pattern step of
+    all the slice's queries is one set of launches over a concatenated table.
+    The native autotuner first picks the probe width and the gather mapping.
+    The steps' algorithmic bytes, counted from the device's own row counts in
+    a separate pass, give the batch's roofline;
   * the same stream per query (one host sync per pattern) for comparison;
   * open-loop Poisson serving replay, continuous batching vs per-query, p50/p99;
-  * the C oracle timed on a bounded sample on the host as the CPU baseline.
+  * the C oracle on the host cores as the CPU baseline: one worker per thread
+    pulling queries from a shared queue over the timed slices, and the single
+    queries on one thread and split over all threads (mt_factor).
 
 `--workload powerlaw`: the power-law graph (seed 7, 100M vertices, 1.2B
 triples, 256 Zipf(1.1) predicates, Pareto(2) out-degree, 64 types) with the
 star / 5-hop path / snowflake stream, starts drawn in proportion to degree.
-`--gstore-budget-gb` below the store size makes segments swap from pinned
-host memory.  The serving replay runs at the offered rates given by `--rates`.
+`--gstore-budget-gb` below the store size makes segments swap from host
+memory.  The serving replay runs at the offered rates given by `--rates`.
 
 Every timed result is checked against the oracle: bit-exact row multisets for
-the single queries, row counts and order-independent checksums for the
-stream.  Prints one JSON line.
+the single queries and for the first 512 batched queries, row counts and
+order-independent row fingerprints for every batched and served query the
+oracle's time budget covers.  Prints one JSON line.
 """
 import argparse
 import json
@@ -99,13 +106,14 @@ def make_workload(args, wg, Q):
     t0 = time.time()
     if args.workload == "university":
         tri = wg.gen_university(args.seed, args.univ)
-        singles = {name: getattr(Q, name)() for name in ("lubm_q1", "type_start2")}
+        singles = {name: getattr(Q, name)() for name in ("lubm_q7", "type_start2", "lubm_q1")}
         desc = ("university graph, %d universities (seed %d, %d triples): %d-query template stream batched "
-                "per step; Q1 lubm_q1 shape, Q2 type-start" % (args.univ, args.seed, len(tri), args.nq))
+                "per step, a distinct slice of the stream each step; single queries Q1 lubm_q7 (6-pattern "
+                "cyclic), Q2 type-start, and lubm_q1" % (args.univ, args.seed, len(tri), args.nq))
         cfg = {"universities": args.univ}
 
-        def stream(store):
-            return Q.template_stream(store, Q.univ_templates(), args.nq, seed=args.seed + 100)
+        def stream(store, n):
+            return Q.template_stream(store, Q.univ_templates(), n, seed=args.seed + 100)
     else:
         tri = wg.gen_powerlaw(args.seed, args.pl_vertices, args.pl_triples)
         singles = {}
@@ -114,8 +122,8 @@ def make_workload(args, wg, Q):
                     args.seed, args.pl_vertices, len(tri), args.nq))
         cfg = {"vertices": args.pl_vertices}
 
-        def stream(store):
-            return Q.powerlaw_stream(tri, Q.powerlaw_templates(), args.nq, seed=args.seed + 100)
+        def stream(store, n):
+            return Q.powerlaw_stream(tri, Q.powerlaw_templates(), n, seed=args.seed + 100)
     log("generated %d triples in %.1fs" % (len(tri), time.time() - t0))
     cfg.update({"triples": int(len(tri)), "queries_per_step": args.nq})
     return tri, singles, stream, desc, cfg
@@ -237,6 +245,97 @@ def roofline_of(single, args):
             "alg_bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms, "launches": d["launches"]}
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: this process's CPU share."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(env))) if env and env.isdigit() else n
+
+
+def cpu_baseline_and_gate(args, wg, dev, tri, single, slices, tq, rows_t, st_t, serve):
+    """The oracle on the host cores of the same box, as the CPU baseline, and
+    the parity gate of everything timed above:
+      * single queries: exact row multisets; CPU latency on one thread and
+        with the index start split over all threads (mt_factor);
+      * the batched stream: slice after timed slice, the oracle runs the
+        slice on one worker per thread pulling from a shared queue (timed),
+        the device runs it again and its result arena comes back; every
+        query's row count and row fingerprint must match, and the first 512
+        queries' rows must be identical as multisets;
+      * served results: fingerprints equal the oracle's.
+    The oracle's time budget bounds the sample (--cpu-seconds)."""
+    import oracle
+    T = cpu_threads()
+    gate = {}
+    t0 = time.time()
+    orc = oracle.Store(tri)
+    log("oracle store built in %.1fs; %d host threads" % (time.time() - t0, T))
+    for name, r in single.items():
+        t = time.perf_counter()
+        want, _, rc = orc.query(r["_pats"], r["_nv"])
+        r["cpu_latency_us"] = (time.perf_counter() - t) * 1e6
+        t = time.perf_counter()
+        want_mt, _, rc_mt = orc.query_mt(r["_pats"], r["_nv"], T)
+        r["cpu_latency_us_%dthreads" % T] = (time.perf_counter() - t) * 1e6
+        g = oracle.sorted_rows(r["_table"])
+        w = oracle.sorted_rows(want)
+        gate[name] = bool(rc == 0 and rc_mt == 0 and g.shape == w.shape and np.array_equal(g, w)
+                          and np.array_equal(oracle.sorted_rows(want_mt), w))
+    # the stream, slice by slice
+    ora_s = 0.0
+    n = 0
+    counts_ok = cks_ok = True
+    checked_rows = 0
+    ck_of = {}
+    k = 0
+    while k * args.nq < len(tq) and ora_s < args.cpu_seconds:
+        sl = tq[k * args.nq:(k + 1) * args.nq]
+        t = time.perf_counter()
+        o_rows, o_ck, o_st = orc.run_stream(sl, T)
+        ora_s += time.perf_counter() - t
+        res = dev.batch(slices[args.warmup + k], use_graph=True, fetch=True)
+        for i, (tab, _, rc) in enumerate(res):
+            q = k * args.nq + i
+            ck_of[q] = (int(o_ck[i]), int(o_rows[i]))
+            if o_st[i] != 0:
+                counts_ok = False
+                continue
+            if rc != 0:
+                continue  # over the table budget in this round: its rows come in a later round
+            if int(rows_t[q]) != o_rows[i] or tab.shape[0] != o_rows[i]:
+                counts_ok = False
+            if (wg.checksum(tab) if tab.size else 0) != int(o_ck[i]):
+                cks_ok = False
+            if q < 512:
+                want, _, _ = orc.query(*sl[i])
+                if not np.array_equal(oracle.sorted_rows(tab), oracle.sorted_rows(want)):
+                    cks_ok = False
+                checked_rows += 1
+        n += len(sl)
+        k += 1
+    served = 0
+    for s in serve.values():
+        for q in range(min(s["_n"], n)):
+            if s["_st"][q] == 0:
+                served += 1
+                if (int(s["_ck"][q]), int(s["_rows"][q])) != ck_of[q]:
+                    cks_ok = False
+    gate["stream_rows"] = bool(counts_ok)
+    gate["stream_fingerprints"] = bool(cks_ok)
+    # one thread over the first slice, for scale
+    m1 = min(len(tq), 2048)
+    t = time.perf_counter()
+    orc.run_stream(tq[:m1], 1)
+    one = m1 / (time.perf_counter() - t)
+    cpu = {"value": n / ora_s, "unit": "queries/s", "cores": T, "kind": "port",
+           "sample": "the first %d queries of the timed stream (%d slices), oracle engine with %d workers "
+                     "pulling from a shared queue (%.1fs); one worker: %.0f q/s" % (n, k, T, ora_s, one),
+           "single_thread_qps": one}
+    log("cpu oracle: %.0f q/s on %d threads (%.0f on one) over %d queries; %d rows-exact, %d served checked; "
+        "gate %s" % (n / ora_s, T, one, n, checked_rows, served, gate))
+    return cpu, gate
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -286,55 +385,90 @@ def main():
         return
 
     # ---------------------------------------------------------------- batch
-    stream = make_stream(store)
+    # one distinct slice of nq queries per warmup and timed step, so no timed
+    # step replays a batch run before it
+    nslices = args.warmup + args.steps
+    t0 = time.time()
+    stream = make_stream(store, nslices * args.nq)
     qs = [(p, nv) for p, nv, _ in stream]
-    pb = wg.PreparedBatch(qs)  # packed once: the timed calls are the C batch entry point
+    slices = [wg.PreparedBatch(qs[k * args.nq:(k + 1) * args.nq]) for k in range(nslices)]
+    log("stream of %d queries in %d slices packed in %.1fs" % (len(qs), nslices, time.time() - t0))
     if not args.no_tune:
-        pw, rpw, tms = dev.autotune(pb, use_graph=not args.no_graph)
+        pw, rpw, tms = dev.autotune(slices[0], use_graph=not args.no_graph)
         log("autotune: probe_width=%d rows_per_wave=%d (%.3f ms per batch in the sweep)" % (pw, rpw, tms))
         out["tuning"] = {"probe_width": pw, "rows_per_wave": rpw, "sweep_ms_per_batch": tms}
-    # the whole job in rounds: all queries, then those whose results did not fit
-    # the rest of the result arena, until a round completes none (their results
-    # alone exceed the arena).  As on the per-query path, each round's results
-    # are left on the device; the next round reuses the arena.
-    chain = [(pb, np.arange(len(qs)))]
-    rows_b, cols_b, st_b = dev.batch(pb, use_graph=not args.no_graph, fetch=False)
-    while True:
-        idx = chain[-1][1]
-        fail = idx[st_b[idx] == wg.WK_E_NOMEM_BUDGET]
-        if len(fail) == 0 or len(fail) == len(idx):
-            break
-        sub = wg.PreparedBatch([qs[i] for i in fail])
-        chain.append((sub, fail))
-        r2, c2, s2 = dev.batch(sub, use_graph=not args.no_graph, fetch=False)
-        rows_b[fail], cols_b[fail], st_b[fail] = r2, c2, s2
+    rows_all = np.zeros(len(qs), np.uint64)
+    st_all = np.zeros(len(qs), np.int32)
+    rounds_all = []
 
-    def run_job():
-        for k, (p_r, idx_r) in enumerate(chain):
-            r_r, c_r, s_r = dev.batch(p_r, use_graph=not args.no_graph, fetch=False)
-            if k == 0:
-                rows_j, cols_j, st_j = r_r, c_r, s_r
-            else:
-                rows_j[idx_r], cols_j[idx_r], st_j[idx_r] = r_r, c_r, s_r
-        return rows_j, cols_j, st_j
+    def run_slice(k):
+        """The whole job of slice k: all its queries, then those whose results
+        did not fit the rest of the result arena, in further rounds that reuse
+        the arena, until a round completes none (their results alone exceed
+        it).  Results stay on the device, as on the per-query path."""
+        base = k * args.nq
+        r, _, st = dev.batch(slices[k], use_graph=not args.no_graph, fetch=False)
+        idx = np.arange(args.nq)
+        nr = 1
+        while True:
+            fail = idx[st[idx] == wg.WK_E_NOMEM_BUDGET]
+            if len(fail) == 0 or len(fail) == len(idx):
+                break
+            sub = wg.PreparedBatch([qs[base + i] for i in fail])
+            r2, _, s2 = dev.batch(sub, use_graph=not args.no_graph, fetch=False)
+            r[fail], st[fail] = r2, s2
+            idx = fail
+            nr += 1
+        rows_all[base:base + args.nq] = r
+        st_all[base:base + args.nq] = st
+        return nr
 
-    for _ in range(args.warmup):
-        rows_b, cols_b, st_b = run_job()
+    for k in range(args.warmup):
+        run_slice(k)
     dev.sync()
     dev.reset_kernel_stats()
     t = time.perf_counter()
-    for _ in range(args.steps):
-        rows_b, cols_b, st_b = run_job()
+    for k in range(args.warmup, nslices):
+        rounds_all.append(run_slice(k))
     dev.sync()
     batch_s = (time.perf_counter() - t) / args.steps
-    out["batch_rounds"] = len(chain)
+    timed = slice(args.warmup * args.nq, nslices * args.nq)
+    out["batch_rounds_max"] = int(max(rounds_all))
     # host phases of wk_batch_run (the device's profile mode times them)
     batch_host = {k[5:]: v["total_ms"] / args.steps for k, v in dev.kernel_stats().items()
                   if k.startswith("host:")}
     out["batch_host_ms"] = batch_host
     if batch_host:
         log("batch host phases (ms per batch): " + ", ".join("%s %.3f" % kv for kv in sorted(batch_host.items())))
-    # the same job with every round's result arena copied to pinned host
+    ok_b = st_all[timed] == 0
+    qps_batched = args.nq / batch_s
+    log("batched: %.3f ms per %d-query slice (up to %d rounds) -> %.0f q/s (%d ok, %d over the table budget, "
+        "%d result rows over %d slices)" % (batch_s * 1e3, args.nq, max(rounds_all), qps_batched, ok_b.sum(),
+                                           (st_all[timed] == wg.WK_E_NOMEM_BUDGET).sum(),
+                                           int(rows_all[timed][ok_b].sum()), args.steps))
+    assert ((st_all == 0) | (st_all == wg.WK_E_NOMEM_BUDGET)).all(), np.unique(st_all)
+
+    # algorithmic bytes of the timed slices' pattern steps, from the device's
+    # own row counts, counted in a pass of its own (the counting reads the
+    # step descriptors back after each batch)
+    dev.set_profile(False)
+    dev.batch_traffic(True)
+    for k in range(args.warmup, nslices):
+        run_slice(k)
+    alg_bytes, alg_edges, nb = dev.batch_traffic(False)
+    dev.set_profile(True)
+    per_step = alg_bytes / args.steps
+    achieved = per_step / batch_s / 1e9
+    out["roofline_batch"] = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "alg_bytes_per_step": per_step, "edges_per_step": alg_edges / args.steps, "ms_per_step": batch_s * 1e3,
+        "edges_per_s": alg_edges / args.steps / batch_s,
+        "note": "whole batched step (all launches and host work) against the per-pattern byte formulas; "
+                "a batch of selective queries is bound by launch and dependent-load latency, not HBM bandwidth"}
+    log("batch roofline: %.1f MB of algorithmic bytes per step -> %.0f GB/s = %.2f%% of peak" % (
+        per_step / 1e6, achieved, 100 * achieved / HBM_PEAK_GBS))
+
+    # the same slices with every round's result arena copied to pinned host
     # memory after the round, as a caller that wants the rows on the host
     # does: the PCIe-inclusive rate, reported beside the HBM-resident one
     L = wg.lib()
@@ -357,42 +491,51 @@ def main():
                                       wg.C.byref(used)))
         return used.value * 4
 
-    def run_job_fetch():
+    def run_slice_fetch(k):
+        base = k * args.nq
         nbytes = 0
-        for p_r, _ in chain:
-            dev.batch(p_r, use_graph=not args.no_graph, fetch=False)
+        _, _, st = dev.batch(slices[k], use_graph=not args.no_graph, fetch=False)
+        nbytes += fetch_arena()
+        idx = np.arange(args.nq)
+        while True:
+            fail = idx[st[idx] == wg.WK_E_NOMEM_BUDGET]
+            if len(fail) == 0 or len(fail) == len(idx):
+                break
+            _, _, s2 = dev.batch(wg.PreparedBatch([qs[base + i] for i in fail]), use_graph=not args.no_graph,
+                                 fetch=False)
             nbytes += fetch_arena()
+            st[fail] = s2
+            idx = fail
         return nbytes
 
-    run_job_fetch()
+    run_slice_fetch(0)
     t = time.perf_counter()
-    for _ in range(args.steps):
-        fetched = run_job_fetch()
+    fetched = 0
+    for k in range(args.warmup, nslices):
+        fetched += run_slice_fetch(k)
     dev.sync()
     fetch_s = (time.perf_counter() - t) / args.steps
     out["pcie_inclusive"] = {"queries_per_s": args.nq / fetch_s, "ms_per_step": fetch_s * 1e3,
-                             "result_bytes_per_step": fetched,
+                             "result_bytes_per_step": fetched / args.steps,
                              "note": "each round's result arena copied to pinned host memory"}
-    log("batched with results copied to the host: %.3f ms per job (%.1f MB) -> %.0f q/s"
-        % (fetch_s * 1e3, fetched / 1e6, args.nq / fetch_s))
+    log("batched with results copied to the host: %.3f ms per slice (%.1f MB) -> %.0f q/s"
+        % (fetch_s * 1e3, fetched / args.steps / 1e6, args.nq / fetch_s))
     host_buf[0] = None
     hip.hipHostFree(host_ptr[0])
-    ok_b = st_b == 0
-    qps_batched = args.nq / batch_s
-    log("batched: %.3f ms per %d queries in %d round(s) -> %.0f q/s (%d ok, %d over the table budget, "
-        "%d result rows)" % (batch_s * 1e3, args.nq, len(chain), qps_batched, ok_b.sum(),
-                             (st_b == wg.WK_E_NOMEM_BUDGET).sum(), int(rows_b[ok_b].sum())))
-    assert ((st_b == 0) | (st_b == wg.WK_E_NOMEM_BUDGET)).all(), np.unique(st_b)
     if args.batch_only:
         print(json.dumps({"metric": "batched queries/s", "value": qps_batched, "ms_per_step": batch_s * 1e3,
-                          "tuning": out.get("tuning"), "config": cfg}))
+                          "tuning": out.get("tuning"), "roofline_batch": out["roofline_batch"], "config": cfg}))
         dev.close()
         return
+    # the timed slices' queries, in stream order, with their batched results
+    tq = qs[args.warmup * args.nq:]
+    rows_t, st_t = rows_all[timed], st_all[timed]
+
     # per-query execution of the same stream (bounded sample), untimed per kernel
     dev.set_profile(False)
     npq = min(args.nq, 1024)
     t = time.perf_counter()
-    for pats, nv in qs[:npq]:
+    for pats, nv in tq[:npq]:
         dev.query(pats, nv, fetch=False)
     pq_s = (time.perf_counter() - t) / npq
     qps_perq = 1.0 / pq_s
@@ -408,12 +551,12 @@ def main():
         else:
             plan = [("per_query", 0.7 * qps_perq), ("batched", 10 * 0.7 * qps_perq)]
         for mode, rate in plan:
-            n = min(len(qs), int(rate * 2.0) + 256)
+            n = min(len(tq), int(rate * 2.0) + 256)
             arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
             dev.reset_kernel_stats()
             # only the continuous-batching loop keeps its (host-side) iteration timer
             dev.set_profile(mode != "per_query")
-            r = dev.serve(qs[:n], arr, max_inflight=4096, per_query=(mode == "per_query"))
+            r = dev.serve(tq[:n], arr, max_inflight=4096, per_query=(mode == "per_query"))
             phases = {k[6:]: {"count": v["launches"], "ms": round(v["total_ms"], 3)}
                       for k, v in dev.kernel_stats().items() if k.startswith("serve:")}
             lat = r["latency_us"]
@@ -422,7 +565,7 @@ def main():
                            "p50_us": float(np.percentile(lat[ok], 50)) if ok.any() else None,
                            "p99_us": float(np.percentile(lat[ok], 99)) if ok.any() else None,
                            "ok": int(ok.sum()), "iterations": int(r["batches"]), "phases": phases,
-                           "_ck": r["checksum"], "_st": r["status"], "_n": n}
+                           "_ck": r["checksum"], "_st": r["status"], "_rows": r["rows"], "_n": n}
             log("serve %s @ %.0f q/s offered: %.0f q/s, p50 %s us, p99 %s us, %d ok; %s" % (
                 mode, rate, n / r["wall_s"], serve[mode]["p50_us"], serve[mode]["p99_us"], ok.sum(), phases))
 
@@ -431,7 +574,7 @@ def main():
     # binary archives, as the proxy sends them) in, reply bundles with the
     # projected rows out, one query at a time
     nag = min(npq, 256)
-    bundles = [wg.wire_encode(wg.query_msg(p, nv, [-(i + 1) for i in range(nv)])) for p, nv in qs[:nag]]
+    bundles = [wg.wire_encode(wg.query_msg(p, nv, [-(i + 1) for i in range(nv)])) for p, nv in tq[:nag]]
     t = time.perf_counter()
     replies = [dev.agent_execute(b) for b in bundles]
     ag_s = (time.perf_counter() - t) / nag
@@ -439,7 +582,7 @@ def main():
     for i in range(nag):
         m = wg.wire_decode(replies[i])
         replies[i] = None
-        want = int(rows_b[i]) if st_b[i] == 0 else None
+        want = int(rows_t[i]) if st_t[i] == 0 else None
         assert m["status_code"] == 0 and (want is None or m["row_num"] == want), (i, m["status_code"])
         ag_rows += m["row_num"]
     del replies
@@ -449,51 +592,16 @@ def main():
     log("agent (bundle in, bundle out): %.1f us/query -> %.0f q/s, %d reply rows"
         % (ag_s * 1e6, 1.0 / ag_s, ag_rows))
 
-
     # ------------------------------------------------------ CPU baseline + gate
     cpu = None
     gate = {}
     if not args.no_cpu:
-        import oracle
-        t0 = time.time()
-        orc = oracle.Store(tri)
-        log("oracle store built in %.1fs" % (time.time() - t0))
-        for name, r in single.items():
-            t = time.perf_counter()
-            want, _, rc = orc.query(r["_pats"], r["_nv"])
-            r["cpu_latency_us"] = (time.perf_counter() - t) * 1e6
-            g = oracle.sorted_rows(r["_table"])
-            w = oracle.sorted_rows(want)
-            gate[name] = bool(rc == 0 and g.shape == w.shape and np.array_equal(g, w))
-        # stream sample: time the oracle until the budget is spent, checking
-        # each result the device returned (queries over the table budget have
-        # no device rows to check)
-        t = time.perf_counter()
-        n = checked = 0
-        ok = True
-        while n < len(qs) and time.perf_counter() - t < args.cpu_seconds:
-            pats, nv = qs[n]
-            want, _, rc = orc.query(pats, nv)
-            ck = wg.checksum(want) if want.size else 0
-            if rc != 0 or (st_b[n] == 0 and int(rows_b[n]) != want.shape[0]):
-                ok = False
-            for s in serve.values():
-                if n < s["_n"] and s["_st"][n] == 0:
-                    ok = ok and int(s["_ck"][n]) == ck
-                    checked += 1
-            n += 1
-        cpu_s = time.perf_counter() - t
-        gate["stream_sample"] = ok
-        cpu = {"value": n / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": "first %d queries of the %d-query stream on the same graph, oracle engine "
-                         "single-threaded (%.1fs); %d served results checked" % (n, args.nq, cpu_s, checked)}
-        log("cpu oracle: %.0f q/s over %d queries; gate %s" % (n / cpu_s, n, gate))
-
+        cpu, gate = cpu_baseline_and_gate(args, wg, dev, tri, single, slices, tq, rows_t, st_t, serve)
     for r in single.values():
         for k in ("_table", "_pats", "_nv", "_v2c"):
             r.pop(k, None)
     for r in serve.values():
-        for k in ("_ck", "_st", "_n"):
+        for k in ("_ck", "_st", "_rows", "_n"):
             r.pop(k, None)
 
     out.update({
@@ -501,7 +609,7 @@ def main():
         "ms_per_step": batch_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic (seeded %s generator)" % args.workload,
         "config": dict({"workload": desc}, **cfg),
-        "batch_ok": int(ok_b.sum()), "batch_result_rows": int(rows_b[ok_b].sum()),
+        "batch_ok": int(ok_b.sum()), "batch_result_rows": int(rows_t[ok_b].sum()),
         "per_query_qps": qps_perq, "batched_vs_per_query": qps_batched / qps_perq,
         "single_query": single,
         "edges_per_s": (edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values())) if single else None,

@@ -307,6 +307,17 @@ typedef struct {
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph,
                  wk_batch_result* out);
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems);
+/* Measurement (no counterpart in the reference).  With enable != 0, each
+ * later wk_batch_run reads its pattern steps' descriptors back once it has
+ * completed and adds their algorithmic bytes to a running total, by the
+ * per-pattern formulas over the device-side row counts (R input rows of C
+ * columns, E output rows of Co columns): an index or constant start
+ * 16 + 8E; known_to_unknown 4RC + 16R + 4E + 4E*Co; known_to_known and
+ * known_to_const 4RC + 16R + 4R + 4E*Co (one edge word per membership
+ * test).  `edges` adds E for expansions and R for membership tests.  Returns
+ * the totals since the last call and resets them; batches counts the
+ * batched launches (sub-batches included) they cover. */
+int wk_batch_traffic(wk_dev* dev, int32_t enable, double* alg_bytes, double* edges, uint64_t* batches);
 /* Autotuner (no counterpart in the reference): times the batch of nq
  * queries under every probe width {1,2,4,8} x gather mapping {0,1,4,16,64}
  * and keeps the fastest on the handle.  Outputs may be null. */

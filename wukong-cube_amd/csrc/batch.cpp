@@ -26,6 +26,35 @@ void launch_batch_steps(wk_dev* dev, uint32_t S, uint32_t nq, bool timed) {
   }
 }
 
+// Algorithmic bytes of the S pattern steps of the batch just run, from the
+// row counts its scans left in the step descriptors.
+int count_traffic(wk_dev* dev, uint32_t S, uint32_t nq) {
+  const size_t qc = dev->nq_cap;
+  if (hip_failed(hipMemcpy(dev->h_desc, dev->d_desc, ((S - 1) * qc + nq) * sizeof(StepDesc), hipMemcpyDeviceToHost)))
+    return WK_E_HIP;
+  double bytes = 0, edges = 0;
+  for (uint32_t k = 0; k < S; k++)
+    for (uint32_t i = 0; i < nq; i++) {
+      const StepDesc& d = dev->h_desc[k * qc + i];
+      if (d.op == OP_SKIP || d.in_rows == 0) continue;
+      const double R = double(d.in_rows), C = double(d.in_cols), E = double(d.out_rows), Co = double(d.out_cols);
+      if (d.op == OP_EXPAND && d.key_mode == KEY_CONST) {
+        bytes += 16 + 8 * E;
+        edges += E;
+      } else if (d.op == OP_EXPAND) {
+        bytes += 4 * R * C + 16 * R + 4 * E + 4 * E * Co;
+        edges += E;
+      } else {
+        bytes += 4 * R * C + 16 * R + 4 * R + 4 * E * Co;
+        edges += R;
+      }
+    }
+  dev->traffic_bytes += bytes;
+  dev->traffic_edges += edges;
+  dev->traffic_batches++;
+  return WK_OK;
+}
+
 }  // namespace
 
 int split_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
@@ -177,6 +206,10 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     return fail(WK_E_HIP);
   unpin_segments(dev, segs);
   t = host_phase(dev, "host:batch_wait", t);
+  if (dev->traffic_on) {
+    int trc = count_traffic(dev, S, nq);
+    if (trc) return trc;
+  }
   bool overflow = false;
   for (uint32_t k = 0; k < S; k++) overflow |= dev->h_state[k].status != 0;
   if (overflow) {

@@ -701,6 +701,17 @@ int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
   return WK_OK;
 }
 
+int wk_batch_traffic(wk_dev* dev, int32_t enable, double* alg_bytes, double* edges, uint64_t* batches) {
+  if (!dev) return WK_E_INVALID_ARG;
+  if (alg_bytes) *alg_bytes = dev->traffic_bytes;
+  if (edges) *edges = dev->traffic_edges;
+  if (batches) *batches = dev->traffic_batches;
+  dev->traffic_bytes = dev->traffic_edges = 0;
+  dev->traffic_batches = 0;
+  dev->traffic_on = enable != 0;
+  return WK_OK;
+}
+
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {
   if (!dev) return WK_E_INVALID_ARG;
   if (used_elems) *used_elems = dev->batch_used;

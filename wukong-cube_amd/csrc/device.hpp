@@ -138,6 +138,10 @@ struct wk_dev {
   uint32_t last_cols = 0;
   // last batch
   uint64_t batch_used = 0;
+  // batched-step traffic accounting (wk_batch_traffic)
+  bool traffic_on = false;
+  double traffic_bytes = 0, traffic_edges = 0;
+  uint64_t traffic_batches = 0;
   // last agent reply bundle, and the agent's query ids
   std::vector<uint8_t> agent_reply;
   int32_t agent_qid = 0;

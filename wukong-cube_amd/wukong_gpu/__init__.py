@@ -182,6 +182,7 @@ SIGNATURES = {
     "wk_dev_autotune": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "wk_batch_download": (C.c_int, [P, U32P, C.c_uint64, U64P]),
+    "wk_batch_traffic": (C.c_int, [P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), U64P]),
     "wk_serve_replay": (C.c_int, [P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
                                   C.POINTER(ServeOpts), C.POINTER(ServeResult)]),
     "wk_serve_result_rows": (C.c_int, [P, C.c_uint32, U32P, C.c_uint64, U64P, U32P]),
@@ -674,6 +675,13 @@ class Device:
             t = arena[o:o + r * c].reshape(r, c) if c else np.zeros((0, 0), np.uint32)
             out.append((t.copy(), list(v2c[i * MAX_VARS:i * MAX_VARS + pb.nvars[i]]), int(status[i])))
         return out
+
+    def batch_traffic(self, enable):
+        """(algorithmic bytes, edges, batches) counted since the last call;
+        `enable` turns the counting of later batches on or off."""
+        b, e, n = C.c_double(), C.c_double(), C.c_uint64()
+        _check(lib().wk_batch_traffic(self._h, 1 if enable else 0, C.byref(b), C.byref(e), C.byref(n)))
+        return b.value, e.value, n.value
 
     def autotune(self, queries, use_graph=True):
         """Native sweep of probe width x gather mapping on a batch; keeps the fastest.
