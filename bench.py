@@ -90,6 +90,8 @@ def parse_args():
     ap.add_argument("--probe-width", type=int, default=0,
                     help="lanes per key in the probe (1, 2, 4, 8; 0 = library default)")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="gather mapping: 0 = output-balanced")
+    ap.add_argument("--dense-index", type=int, default=0,
+                    help="direct key indexes of dense segments: 0 = build (default), -1 = probe the buckets only")
     ap.add_argument("--no-graph", action="store_true", help="launch batched steps directly (no hipGraph)")
     ap.add_argument("--single-only", action="store_true",
                     help="only the single-query kernels and their roofline (the launches a kernel trace "
@@ -359,12 +361,13 @@ def main():
     log("host gstore built in %.1fs: %s" % (time.time() - t0, info))
     dev = wg.Device(store, gstore_budget=int(args.gstore_budget_gb * (1 << 30)),
                     table_budget=int(args.table_budget_gb * (1 << 30)), profile=1,
-                    probe_width=args.probe_width, rows_per_wave=args.rows_per_wave)
+                    probe_width=args.probe_width, rows_per_wave=args.rows_per_wave, dense_index=args.dense_index)
     t0 = time.time()
     dev.load_all()
     st = dev.stats()
-    log("gstore loaded in %.1fs (%.2f GB H2D, %d key / %d value blocks free)" % (
-        time.time() - t0, st["h2d_bytes"] / 1e9, st["key_blocks_free"], st["value_blocks_free"]))
+    log("gstore loaded in %.1fs (%.2f GB H2D, %d key / %d value blocks free; direct key indexes for %d "
+        "segments, %.2f GB)" % (time.time() - t0, st["h2d_bytes"] / 1e9, st["key_blocks_free"],
+                                st["value_blocks_free"], st["dense_segments"], st["dense_bytes"] / 1e9))
 
     out = {"metric": "queries/sec + p50/p99 latency (batched vs per-query GPU); edges/s; HBM GB/s"}
 

@@ -170,6 +170,11 @@ typedef struct {
   int32_t probe_width;        /* lanes sharing one key's bucket probe: 1,2,4,8 (0 -> 2) */
   int32_t rows_per_wave;      /* gather mapping: 1,4,16,64 or 0 = balanced search */
   int32_t profile;            /* 1: time every kernel with HIP events */
+  int32_t dense_index;        /* direct key indexes of segments whose key vids are dense
+                                 (at least one key per 16 vids of the segment's range; one 8-B
+                                 ptr word per vid, probed instead of the buckets): 0 = build
+                                 them within a quarter of free HBM (and of gstore_budget when
+                                 set), most keys first; -1 = never */
 } wk_dev_opts;
 
 typedef struct {
@@ -180,6 +185,8 @@ typedef struct {
   uint64_t value_blocks_total, value_blocks_free;
   uint64_t table_capacity_elems;
   uint64_t queries_run;
+  uint64_t dense_segments;    /* segments with a direct key index */
+  uint64_t dense_bytes;
 } wk_dev_stats;
 
 /* Per-kernel timing (opts.profile = 1): name, launches, total ms on the

@@ -83,12 +83,16 @@ def test_type_index_expansion(univ10_dev, univ10_oracle):
     assert t.shape[0] > 100
 
 
+@pytest.mark.parametrize("dense", [0, -1])
 @pytest.mark.parametrize("seed", range(6))
-def test_random_graphs(seed):
+def test_random_graphs(seed, dense):
+    """Random graphs and queries; probes through the direct key indexes
+    (dense = 0) and through the hash buckets only (dense = -1)."""
     tri, info = random_graph(seed, tpid_objects=(seed % 2 == 0))
     orc = oracle.Store(tri)
     st = wg.Store(tri, load_factor_pct=95 if seed % 3 == 0 else 0)  # 95%: long bucket chains
-    dev = wg.Device(st, table_budget=64 << 20)
+    dev = wg.Device(st, table_budget=64 << 20, dense_index=dense)
+    assert (dev.stats()["dense_segments"] > 0) == (dense == 0)
     rng = np.random.default_rng(100 + seed)
     for _ in range(60):
         pats, nv = random_query(rng, info)
@@ -448,3 +452,21 @@ def test_final_process_empty_and_errors(univ10_dev):
             univ10_dev.final_process(dt, [0, 1, 2, -1], req, orders=orders)
         assert ei.value.code == code
     dt.free()
+
+
+def test_direct_key_index_and_buckets_agree(univ10_store, univ10_oracle):
+    """The same queries, per query and batched, with the direct key indexes
+    of the dense segments (students' memberOf/takesCourse, rdf:type OUT, ...)
+    and with every probe through the hash buckets: both equal the oracle."""
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 300, seed=41)
+    qs = [(p, nv) for p, nv, _ in stream] + [f() for f in (Q.chain3, Q.type_start2, Q.lubm_q1, Q.lubm_q7)]
+    for dense in (0, -1):
+        dev = wg.Device(univ10_store, table_budget=256 << 20, dense_index=dense)
+        s = dev.stats()
+        assert (s["dense_segments"] >= 4) if dense == 0 else (s["dense_segments"] == 0)
+        for pats, nv in qs[-4:]:
+            check_query(dev, univ10_oracle, pats, nv)
+        for (pats, nv), (t, v2c, rc) in zip(qs, dev.batch(qs, use_graph=True)):
+            want, wv, wrc = univ10_oracle.query(pats, nv)
+            assert rc == wrc and v2c == wv and same_rows(t, want), (dense, pats)
+        dev.close()

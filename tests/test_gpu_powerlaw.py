@@ -73,13 +73,14 @@ def test_powerlaw_per_query(pl, pl_stream):
     dev.close()
 
 
-@pytest.mark.parametrize("pw", [1, 2, 8])
-def test_powerlaw_batch_matches_oracle(pl, pl_stream, pw):
+@pytest.mark.parametrize("pw,dense", [(1, 0), (2, 0), (8, 0), (2, -1)])
+def test_powerlaw_batch_matches_oracle(pl, pl_stream, pw, dense):
     """The 600-query stream in one batch (one set of launches per pattern
-    step over a concatenated table), replayed from a hipGraph three times."""
+    step over a concatenated table), replayed from a hipGraph three times,
+    with and without direct key indexes."""
     _, _, store = pl
     qs, _, wants = pl_stream
-    dev = wg.Device(store, table_budget=512 << 20, probe_width=pw)
+    dev = wg.Device(store, table_budget=512 << 20, probe_width=pw, dense_index=dense)
     dev.load_all()
     for rep in range(3):
         res = dev.batch(qs, use_graph=True)

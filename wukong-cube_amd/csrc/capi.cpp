@@ -38,11 +38,52 @@ void free_dev(wk_dev* d) {
   void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state, d->h_qres, d->h_win};
   for (void* p : host_ptrs)
     if (p) (void)hipHostFree(p);
+  for (uint64_t* p : d->dense)
+    if (p) (void)hipFree(p);
   if (d->copy_done) (void)hipEventDestroy(d->copy_done);
   if (d->compute_done) (void)hipEventDestroy(d->compute_done);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   if (d->copy_stream) (void)hipStreamDestroy(d->copy_stream);
   delete d;
+}
+
+// Direct key indexes for the normal segments whose key vids are dense enough
+// (at least one key per 16 vids of their range: 8 B per vid, so at most
+// 128 B per key, one bucket line), those with the most keys first, within a
+// quarter of the HBM still free and, with a gstore budget, a quarter of it.
+// Off with opts.dense_index < 0.
+int alloc_dense(wk_dev* d) {
+  const Store* st = d->st;
+  d->dense.assign(st->segs.size(), nullptr);
+  d->dense_built.assign(st->segs.size(), 0);
+  if (d->opts.dense_index < 0) return WK_OK;
+  size_t free_b = 0, total_b = 0;
+  WK_HIP(hipMemGetInfo(&free_b, &total_b));
+  uint64_t room = free_b / 4;
+  if (d->opts.gstore_budget) room = std::min<uint64_t>(room, d->opts.gstore_budget / 4);
+  std::vector<std::pair<uint64_t, size_t>> cand;  // (keys, segment)
+  for (size_t i = 0; i < st->segs.size(); i++) {
+    const SegMeta& m = st->segs[i];
+    if (m.id.index != 0 || m.num_keys < 64) continue;
+    const uint64_t span = uint64_t(m.vmax) - m.vmin + 1;
+    if (span <= 16 * m.num_keys) cand.emplace_back(m.num_keys, i);
+  }
+  std::sort(cand.rbegin(), cand.rend());
+  for (auto& c : cand) {
+    const SegMeta& m = st->segs[c.second];
+    const uint64_t bytes = (uint64_t(m.vmax) - m.vmin + 1) * sizeof(uint64_t);
+    if (bytes > room) continue;
+    uint64_t* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      break;
+    }
+    room -= bytes;
+    d->dense[c.second] = p;
+    d->stats.dense_segments++;
+    d->stats.dense_bytes += bytes;
+  }
+  return WK_OK;
 }
 
 int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
@@ -84,13 +125,14 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
     SegSlot& s = d->seg[i];
     s.nkb = uint32_t(ceil_div(m.total_buckets(), uint64_t(1) << d->kshift));
     s.nvb = uint32_t(ceil_div(m.num_edges, uint64_t(1) << d->vshift));
+    if (m.total_buckets() >= (uint64_t(1) << 32)) return WK_E_OVERFLOW;
     DevSeg& h = d->h_segs[i];
     h = DevSeg{};
-    h.num_buckets = m.num_buckets;
+    h.num_buckets = uint32_t(m.num_buckets);
     h.nb_magic = bucket_magic(m.num_buckets);
-    h.total_buckets = m.total_buckets();
+    h.total_buckets = uint32_t(m.total_buckets());
     h.edge_start = m.edge_start;
-    h.num_edges = m.num_edges;
+    h.num_edges = uint32_t(m.num_edges);
     h.kb_base = uint32_t(kt);
     h.vb_base = uint32_t(vt);
     h.resident = 0;
@@ -153,6 +195,8 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   WK_HIP(hipMalloc(&d->tile_sum, d->tile_cap * sizeof(uint64_t)));
   WK_HIP(hipMalloc(&d->tile_row, (d->otile_cap + 1) * sizeof(uint64_t)));
   int rc = reserve_steps(d, 8, 64);
+  if (rc) return rc;
+  rc = alloc_dense(d);
   if (rc) return rc;
   d->stats.key_blocks_total = nk;
   d->stats.value_blocks_total = nv;

@@ -215,6 +215,16 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
   };
   hs.kphys = first_if_consecutive(s.kblk);
   hs.vphys = first_if_consecutive(s.vblk);
+  if (dev->dense[si] && !dev->dense_built[si]) {
+    // the direct key index from the buckets just copied (copy stream, so the
+    // compute stream sees it once it waits for the copies)
+    hs.dvmin = m.vmin;
+    hs.dvn = m.vmax - m.vmin + 1;
+    WK_HIP(hipMemsetAsync(dev->dense[si], 0, uint64_t(hs.dvn) * sizeof(uint64_t), dev->copy_stream));
+    launch_dense_build(dev->karena, dev->d_ktab, dev->kshift, hs, dev->dense[si], dev->copy_stream);
+    hs.dense = dev->dense[si];
+    dev->dense_built[si] = 1;
+  }
   hs.resident = 1;
   WK_HIP(hipMemcpyAsync(dev->d_segs + si, dev->h_segs + si, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
@@ -425,9 +435,13 @@ int probe_scan(wk_dev* dev, const StepDesc& sd0, const uint32_t* tin, uint64_t R
   // each launch: key column 4 B + slot 16 B + cnt/eoff 8 B per row for the
   // probe, cnt 4 B + prefix 8 B per row for the row scan
   const double ntiles = std::ceil(Rd / double(kTile));
+  // a key read from a direct index is one 8-B word, so lanes sharing a key
+  // would only repeat it: one lane per key
+  const bool direct = sd.seg >= 0 && dev->h_segs[sd.seg].dense && sd.key_mode != KEY_TYPE_INDEX;
   timed_begin(dev, &ev);
-  launch_probe(a, dev->opts.probe_width, R, dev->stream);
-  timed_end(dev, ev, "k_probe", 28 * Rd);
+  launch_probe(a, direct ? 1 : dev->opts.probe_width, R, dev->stream);
+  // direct key index: 8-B ptr word instead of the 16-B slot
+  timed_end(dev, ev, "k_probe", (direct ? 20 : 28) * Rd);
   timed_begin(dev, &ev);
   launch_scan_tiles(a, dev->stream);
   timed_end(dev, ev, "k_scan_tiles", 16 * ntiles);

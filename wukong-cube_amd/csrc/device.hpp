@@ -85,6 +85,11 @@ struct wk_dev {
   uint32_t* varena = nullptr;
   std::vector<uint32_t> free_k, free_v;
   std::vector<wk::SegSlot> seg;
+  // direct key indexes (null where a segment has none); built on a
+  // segment's first load and kept across evictions (they hold ptr words, not
+  // arena places)
+  std::vector<uint64_t*> dense;
+  std::vector<uint8_t> dense_built;
   wk::DevSeg* h_segs = nullptr;   // pinned mirror
   wk::DevSeg* d_segs = nullptr;
   uint32_t* h_ktab = nullptr;     // pinned mirror

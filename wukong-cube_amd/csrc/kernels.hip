@@ -365,7 +365,16 @@ __device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d,
   const uint64_t key = tix ? make_key(0, kv, DIR_IN) : make_key(kv, d.pid, d.dir);
   const bool valid = s.resident != 0 && (!tix || kv < (1u << NBITS_IDX));
   uint64_t ptr = 0;
-  if (valid && probe<W>(a, s, key, ptr)) {
+  bool hit;
+  if (s.dense && !tix) {
+    // the key's slot word straight from the direct index (one 8-B read)
+    const uint32_t i = kv - s.dvmin;
+    if (valid && i < s.dvn) ptr = s.dense[i];
+    hit = ptr != 0;
+  } else {
+    hit = valid && probe<W>(a, s, key, ptr);
+  }
+  if (hit) {
     const uint64_t sz = ptr_size(ptr);
     const uint64_t off = ptr_off(ptr) - s.edge_start;
     if (d.op == OP_EXPAND) {
@@ -429,6 +438,17 @@ __device__ __forceinline__ void probe_row2(const StepArgs& a, const ProbeDesc& d
   const bool konst = d.key_mode == KEY_CONST, tix = d.key_mode == KEY_TYPE_INDEX;
   const uint32_t kv0 = konst ? d.kstart : row0[d.sc];
   const uint32_t kv1 = konst ? d.kstart : row1[d.sc];
+  if (s.dense && !tix) {
+    // both keys' slot words from the direct index, loaded together at
+    // in-bounds indices and masked after
+    const uint32_t i0 = kv0 - s.dvmin, i1 = kv1 - s.dvmin;
+    const bool in0 = i0 < s.dvn, in1 = i1 < s.dvn;
+    const uint64_t w0 = s.dense[in0 ? i0 : 0], w1 = s.dense[in1 ? i1 : 0];
+    const uint64_t p0 = in0 ? w0 : 0, p1 = in1 ? w1 : 0;
+    finish_row<W>(a, d, s, row0, p0 != 0, p0, c0, eo0);
+    finish_row<W>(a, d, s, row1, p1 != 0, p1, c1, eo1);
+    return;
+  }
   const uint64_t key0 = tix ? make_key(0, kv0, DIR_IN) : make_key(kv0, d.pid, d.dir);
   const uint64_t key1 = tix ? make_key(0, kv1, DIR_IN) : make_key(kv1, d.pid, d.dir);
   uint4 v0[ASSOCIATIVITY / W], v1[ASSOCIATIVITY / W];
@@ -1002,12 +1022,16 @@ __device__ __forceinline__ void add_checksum(const StepArgs& a, uint32_t q, uint
 // a binary search there (a tile of a heavy row stages one entry).  Tiles of
 // many empty rows (selective filters) search the global prefix instead.
 constexpr int kStage = kOutTile + 1;
+constexpr int kRegCols = 8;  // input columns a gather thread holds in registers
 constexpr int kQStage = kBlock;  // queries of a mixed tile staged in LDS
 
 template <bool CK>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   if (a.st->status) return;
-  __shared__ uint64_t s_pfx[kStage];
+  // the tile's source rows: their first output row relative to the tile's
+  // first, pfx - j0, in (-2^28, kOutTile] for every row holding one of the
+  // tile's output rows (a row has fewer than 2^28 edges), clamped above
+  __shared__ int32_t s_rel[kStage];
   __shared__ uint32_t s_eoff[kStage];
   __shared__ uint64_t s_qr0[kQStage];  // first input row of each query of the tile
   __shared__ uint64_t s_qp0[kQStage];  // and its first output row
@@ -1040,7 +1064,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
         const uint64_t pv = a.pfx[gi];
         const uint32_t ev = a.eoff[gi];
         if (i < uint32_t(nr)) {
-          s_pfx[i] = pv;
+          const int64_t r = int64_t(pv - j0);
+          s_rel[i] = int32_t(r < (int64_t(1) << 30) ? r : (int64_t(1) << 30));
           s_eoff[i] = ev;
         }
       }
@@ -1073,32 +1098,78 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       const DevSeg sg = a.segs[expand ? d.seg : 0];
       const bool ck = CK && a.ck && d.final_step;
       uint64_t hsum = 0;  // row hashes of this thread's final rows
-      uint32_t l = 0;  // source rows only move forward as j grows
+      constexpr int kR = kOutTile / kBlock;
+      // each output row's source row in the tile, by independent searches
+      uint32_t ls[kR];
+      bool act[kR];
 #pragma unroll
-      for (int k = 0; k < kOutTile / kBlock; ++k) {
-        const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
-        if (j >= total) continue;
-        uint32_t h = uint32_t(nr);
+      for (int k = 0; k < kR; ++k) {
+        const int32_t jl = k * kBlock + threadIdx.x;
+        act[k] = j0 + uint64_t(jl) < total;
+        uint32_t l = 0, h = uint32_t(nr);
         while (h - l > 1) {
           const uint32_t m = (l + h) >> 1;
-          if (s_pfx[m] <= j) l = m;
+          if (s_rel[m] <= jl) l = m;
           else h = m;
         }
-        const uint64_t g = lo + l;
-        uint32_t* dst = dbase + (j - p0 - a.win_lo) * oc;
-        const uint32_t* src = sbase + (g - r0) * ic;
-        uint64_t rh = row_hash_init(oc);
-        for (uint32_t c = 0; c < ic; ++c) {
-          const uint32_t x = src[c];
-          dst[c] = x;
-          if (ck) rh = row_hash_step(rh, x, c);
+        ls[k] = l;
+      }
+      if (ic <= kRegCols) {
+        // every row's words and edge are loaded before anything is stored:
+        // a store may alias a later load, so loads after it would each wait
+        // for the one before
+        uint32_t v[kR][kRegCols];
+        uint32_t e[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+          const uint32_t* src = sbase + (lo + ls[k] - r0) * ic;  // an input row of the tile
+#pragma unroll
+          for (int c = 0; c < kRegCols; ++c) v[k][c] = uint32_t(c) < ic ? src[c] : 0u;
+          const int32_t jl = k * kBlock + threadIdx.x;
+          e[k] = (expand && act[k]) ? edge_at(a, sg, uint64_t(s_eoff[ls[k]]) + uint64_t(int64_t(jl) - s_rel[ls[k]]))
+                                    : 0u;
         }
-        if (expand) {
-          const uint32_t x = edge_at(a, sg, uint64_t(s_eoff[l]) + (j - s_pfx[l]));
-          dst[ic] = x;
-          if (ck) rh = row_hash_step(rh, x, ic);
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+          if (!act[k]) continue;
+          const uint32_t jl = k * kBlock + threadIdx.x;
+          uint32_t* dst = dbase + (j0 + jl - p0 - a.win_lo) * oc;
+          uint64_t rh = row_hash_init(oc);
+#pragma unroll
+          for (int c = 0; c < kRegCols; ++c) {
+            if (uint32_t(c) < ic) {
+              dst[c] = v[k][c];
+              if (ck) rh = row_hash_step(rh, v[k][c], c);
+            }
+          }
+          if (expand) {
+            dst[ic] = e[k];
+            if (ck) rh = row_hash_step(rh, e[k], ic);
+          }
+          if (ck) hsum += rh;
         }
-        if (ck) hsum += rh;
+      } else {
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+          if (!act[k]) continue;
+          const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+          const uint32_t l = ls[k];
+          const uint64_t g = lo + l;
+          uint32_t* dst = dbase + (j - p0 - a.win_lo) * oc;
+          const uint32_t* src = sbase + (g - r0) * ic;
+          uint64_t rh = row_hash_init(oc);
+          for (uint32_t c = 0; c < ic; ++c) {
+            const uint32_t x = src[c];
+            dst[c] = x;
+            if (ck) rh = row_hash_step(rh, x, c);
+          }
+          if (expand) {
+            const uint32_t x = edge_at(a, sg, uint64_t(s_eoff[l]) + uint64_t(int64_t(j - j0) - s_rel[l]));
+            dst[ic] = x;
+            if (ck) rh = row_hash_step(rh, x, ic);
+          }
+          if (ck) hsum += rh;
+        }
       }
       if (ck) {
         hsum = block_sum<kBlock>(hsum);
@@ -1112,10 +1183,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
         // each branch writes its row itself, so no pointer into LDS or global
         // memory is carried out of it
         if (staged && qstaged) {
+          const int32_t jl = int32_t(j - j0);
           uint32_t l = 0, h = uint32_t(nr);
           while (h - l > 1) {
             const uint32_t m = (l + h) >> 1;
-            if (s_pfx[m] <= j) l = m;
+            if (s_rel[m] <= jl) l = m;
             else h = m;
           }
           const uint64_t g = lo + l;
@@ -1126,18 +1198,20 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
             else qh = m;
           }
           // checksum shares gather in LDS, one global add per query and tile
-          const uint64_t rh = emit_row<CK>(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, s_pfx[l], s_eoff[l]);
+          const uint64_t rh =
+              emit_row<CK>(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, uint64_t(int64_t(j0) + s_rel[l]), s_eoff[l]);
           if (CK && rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
         } else if (staged) {
+          const int32_t jl = int32_t(j - j0);
           uint32_t l = 0, h = uint32_t(nr);
           while (h - l > 1) {
             const uint32_t m = (l + h) >> 1;
-            if (s_pfx[m] <= j) l = m;
+            if (s_rel[m] <= jl) l = m;
             else h = m;
           }
           const uint64_t g = lo + l;
           const uint32_t q = query_in(a.row_start, qlo, qhi, g);
-          const uint64_t rh = emit_row<CK>(a, q, g, j, s_pfx[l], s_eoff[l]);
+          const uint64_t rh = emit_row<CK>(a, q, g, j, uint64_t(int64_t(j0) + s_rel[l]), s_eoff[l]);
           if (CK) add_checksum(a, q, rh);
         } else {
           uint64_t l = lo, h = hi;
@@ -1419,6 +1493,25 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     p[i] = v;
 }
 
+// One thread per bucket of the segment: every key in slots 0..6 writes its
+// ptr word at its vid's place in the direct index.
+__global__ __launch_bounds__(kBlock) void k_dense_build(const Slot* karena, const uint32_t* ktab, uint32_t kshift,
+                                                        DevSeg s, uint64_t* dense) {
+  const uint64_t mask = (uint64_t(1) << kshift) - 1;
+  for (uint64_t b = uint64_t(blockIdx.x) * kBlock + threadIdx.x; b < s.total_buckets;
+       b += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t i = s.kphys != kNoPhys ? (uint64_t(s.kphys) << kshift) + b
+                                          : (uint64_t(ktab[s.kb_base + (b >> kshift)]) << kshift) | (b & mask);
+    const Slot* bk = karena + i * ASSOCIATIVITY;
+    for (int k = 0; k < ASSOCIATIVITY - 1; ++k) {
+      const Slot sl = bk[k];
+      if (sl.key == 0) continue;
+      const uint64_t v = key_vid(sl.key) - s.dvmin;
+      if (v < s.dvn) dense[v] = sl.ptr;
+    }
+  }
+}
+
 // Largest g < R with pfx[g] <= j (pfx[0] = 0 <= j).
 __device__ __forceinline__ uint64_t row_of_output(const uint64_t* pfx, uint64_t R, uint64_t j) {
   uint64_t lo = 0, hi = R;
@@ -1652,6 +1745,12 @@ void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req,
   if (n == 0) return;
   hipLaunchKernelGGL(k_fp_project, dim3(grid_for((n * req.n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
                      table, cols, req, sorted, first, n, out);
+}
+
+void launch_dense_build(const Slot* karena, const uint32_t* ktab, uint32_t kshift, const DevSeg& s, uint64_t* dense,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_build, dim3(grid_for((uint64_t(s.total_buckets) + kBlock - 1) / kBlock, 8192)),
+                     dim3(kBlock), 0, st, karena, ktab, kshift, s, dense);
 }
 
 void launch_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C, uint64_t T, uint32_t n,

@@ -24,19 +24,23 @@ enum : uint32_t { KEY_FROM_ROW = 0, KEY_CONST = 1, KEY_TYPE_INDEX = 2 };
 
 // Segment descriptor as seen by the kernels (device array, one per segment).
 struct DevSeg {
-  uint64_t num_buckets;  // main buckets: hash modulus
-  uint64_t total_buckets;// main + extended: bound on chain links
-  uint64_t edge_start;   // global edge offset of the segment (ptr.off - edge_start = local)
-  uint64_t num_edges;
-  uint64_t nb_magic;     // bucket_magic(num_buckets)
-  uint32_t kb_base;      // first entry in the key-block table
-  uint32_t vb_base;      // first entry in the value-block table
+  uint64_t nb_magic;      // bucket_magic(num_buckets)
+  uint64_t edge_start;    // global edge offset of the segment (ptr.off - edge_start = local)
+  const uint64_t* dense;  // direct key index of a segment whose key vids are dense: the ptr
+                          // word of key vid dvmin + i at dense[i], 0 for no key; or null
+  uint32_t num_buckets;   // main buckets: hash modulus
+  uint32_t total_buckets; // main + extended: bound on chain links
+  uint32_t num_edges;
+  uint32_t kb_base;       // first entry in the key-block table
+  uint32_t vb_base;       // first entry in the value-block table
   uint32_t resident;
-  uint32_t kphys;        // first physical key block when the segment's key blocks are
-                         // consecutive in the arena, else kNoPhys
-  uint32_t vphys;        // same for the value blocks
-  uint32_t pad;
+  uint32_t kphys;         // first physical key block when the segment's key blocks are
+                          // consecutive in the arena, else kNoPhys
+  uint32_t vphys;         // same for the value blocks
+  uint32_t dvmin;         // first vid the direct index covers
+  uint32_t dvn;           // vids it covers (>= 1 when dense is set)
 };
+static_assert(sizeof(DevSeg) == 64, "DevSeg is one 64-byte line");
 constexpr uint32_t kNoPhys = 0xFFFFFFFFu;
 // StepDesc.out_off (and the arena offset in qres) of a final table that did
 // not fit the result arena: the query reports WK_E_NOMEM_BUDGET, the others
@@ -169,6 +173,12 @@ void launch_fp_dedup(const uint32_t* table, uint32_t cols, const ColList& req, c
 void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted,
                        uint64_t first, uint64_t n, uint32_t* out, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+
+// Direct key index of a resident segment s (its buckets at karena through
+// ktab, s.dense/dvmin/dvn set): dense[vid - dvmin] = ptr of every key in the
+// segment's buckets.  dense must be zeroed before.
+void launch_dense_build(const Slot* karena, const uint32_t* ktab, uint32_t kshift, const DevSeg& s, uint64_t* dense,
+                        hipStream_t st);
 
 // Chunk boundaries of a step run in output windows of C rows: for chunk i
 // (output rows [j, min(T, j + C)) with j = j0 + i * C), out[3i] and

@@ -169,6 +169,11 @@ struct SegBuild {
 
 void bucketize(SegBuild& sb, int lf) {
   const uint64_t nk = sb.keys.size();
+  if (nk && sb.meta.id.index == 0) {
+    // keys are built in ascending vid order
+    sb.meta.vmin = uint32_t(key_vid(sb.keys.front()));
+    sb.meta.vmax = uint32_t(key_vid(sb.keys.back()));
+  }
   uint64_t nb = (nk * 100 + uint64_t(ASSOCIATIVITY) * lf - 1) / (uint64_t(ASSOCIATIVITY) * lf);
   if (nb == 0) nb = 1;
   sb.meta.num_keys = nk;

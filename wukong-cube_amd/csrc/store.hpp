@@ -21,6 +21,7 @@ struct SegMeta {
   uint64_t bucket_start = 0; // global bucket offset
   uint64_t num_edges = 0;
   uint64_t edge_start = 0;   // global edge offset
+  uint32_t vmin = 0, vmax = 0;  // smallest and largest key vid (normal segments)
   uint64_t total_buckets() const { return num_buckets + num_ext; }
 };
 

@@ -72,13 +72,14 @@ class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("gstore_budget", C.c_uint64), ("table_budget", C.c_uint64),
                 ("key_block_shift", C.c_uint32), ("value_block_shift", C.c_uint32),
                 ("enable_pipeline", C.c_int32), ("probe_width", C.c_int32),
-                ("rows_per_wave", C.c_int32), ("profile", C.c_int32)]
+                ("rows_per_wave", C.c_int32), ("profile", C.c_int32), ("dense_index", C.c_int32)]
 
 
 class DevStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "h2d_bytes", "segments_loaded", "segments_evicted", "key_blocks_total", "key_blocks_free",
-        "value_blocks_total", "value_blocks_free", "table_capacity_elems", "queries_run")]
+        "value_blocks_total", "value_blocks_free", "table_capacity_elems", "queries_run", "dense_segments",
+        "dense_bytes")]
 
 
 class KernelStat(C.Structure):
@@ -524,11 +525,12 @@ class Device:
     """HBM gstore cache + history tables + expansion kernels on one GPU."""
 
     def __init__(self, store, device=0, gstore_budget=0, table_budget=0, key_block_shift=0,
-                 value_block_shift=0, enable_pipeline=1, probe_width=0, rows_per_wave=0, profile=0):
+                 value_block_shift=0, enable_pipeline=1, probe_width=0, rows_per_wave=0, profile=0,
+                 dense_index=0):
         self.store = store
         self._h = P()
         o = DevOpts(device, gstore_budget, table_budget, key_block_shift, value_block_shift,
-                    enable_pipeline, probe_width, rows_per_wave, profile)
+                    enable_pipeline, probe_width, rows_per_wave, profile, dense_index)
         _check(lib().wk_dev_create(store.handle, C.byref(o), C.byref(self._h)), "wk_dev_create")
 
     def close(self):
