@@ -225,9 +225,11 @@ def test_gstore_swapping(univ10_store, univ10_oracle):
     """A gstore budget below the store size evicts and reloads segments."""
     info = univ10_store.info()
     total = info["num_buckets"] * 128 + info["num_values"] * 4
-    # the largest single segment (type OUT keys, index IN values) still fits
+    # the largest single segment (type OUT keys, index IN values) still fits;
+    # no direct indexes or type bitmaps, so that the type filters read the
+    # type segment as well
     dev = wg.Device(univ10_store, gstore_budget=total * 2 // 5, key_block_shift=10, value_block_shift=14,
-                    table_budget=256 << 20)
+                    table_budget=256 << 20, dense_index=-1)
     for f in (Q.chain3, Q.type_start2, Q.lubm_q1, Q.chain3):
         pats, nv = f()
         check_query(dev, univ10_oracle, pats, nv)

@@ -40,6 +40,8 @@ void free_dev(wk_dev* d) {
     if (p) (void)hipHostFree(p);
   for (uint64_t* p : d->dense)
     if (p) (void)hipFree(p);
+  for (uint32_t* p : d->tbits_words) (void)hipFree(p);
+  if (d->d_tbits) (void)hipFree(d->d_tbits);
   if (d->copy_done) (void)hipEventDestroy(d->copy_done);
   if (d->compute_done) (void)hipEventDestroy(d->compute_done);
   if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -56,11 +58,50 @@ int alloc_dense(wk_dev* d) {
   const Store* st = d->st;
   d->dense.assign(st->segs.size(), nullptr);
   d->dense_built.assign(st->segs.size(), 0);
+  d->type_slot.assign(size_t(1) << NBITS_IDX, -1);
   if (d->opts.dense_index < 0) return WK_OK;
   size_t free_b = 0, total_b = 0;
   WK_HIP(hipMemGetInfo(&free_b, &total_b));
   uint64_t room = free_b / 4;
   if (d->opts.gstore_budget) room = std::min<uint64_t>(room, d->opts.gstore_budget / 4);
+  // instance bitmaps of the types first (a bit per vid of each type's range)
+  const size_t nt = st->type_ids.size();
+  std::vector<std::vector<uint32_t>> words(nt);
+  std::vector<TypeBits> tb(nt);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t k = 0; k < nt; k++) {
+    uint64_t n = 0;
+    const uint32_t* inst = st->get_index(st->type_ids[k], DIR_IN, n);
+    if (!n) continue;
+    tb[k].vmin = inst[0];
+    tb[k].nbits = inst[n - 1] - inst[0] + 1;  // the list is ascending
+    words[k].assign((uint64_t(tb[k].nbits) + 31) / 32, 0u);
+    for (uint64_t i = 0; i < n; i++) {
+      const uint32_t b = inst[i] - inst[0];
+      words[k][b >> 5] |= 1u << (b & 31);
+    }
+  }
+  std::vector<TypeBits> slots;
+  for (size_t k = 0; k < nt; k++) {
+    const uint64_t bytes = words[k].size() * sizeof(uint32_t);
+    if (!bytes || bytes > room) continue;
+    uint32_t* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      break;
+    }
+    WK_HIP(hipMemcpy(p, words[k].data(), bytes, hipMemcpyHostToDevice));
+    room -= bytes;
+    d->tbits_words.push_back(p);
+    tb[k].words = p;
+    d->type_slot[st->type_ids[k]] = int32_t(slots.size());
+    slots.push_back(tb[k]);
+    d->stats.dense_bytes += bytes;
+  }
+  if (!slots.empty()) {
+    WK_HIP(hipMalloc(&d->d_tbits, slots.size() * sizeof(TypeBits)));
+    WK_HIP(hipMemcpy(d->d_tbits, slots.data(), slots.size() * sizeof(TypeBits), hipMemcpyHostToDevice));
+  }
   std::vector<std::pair<uint64_t, size_t>> cand;  // (keys, segment)
   for (size_t i = 0; i < st->segs.size(); i++) {
     const SegMeta& m = st->segs[i];
@@ -603,6 +644,12 @@ static int known_step(wk_dev* dev, const wk_table* in, uint32_t start_col, uint3
   }
   if (op == OP_FILTER_KNOWN) sd.ec = end;
   if (op == OP_FILTER_CONST) sd.kend = end;
+  if (op == OP_FILTER_CONST && pid == TYPE_ID && dir == WK_OUT && end < (1u << NBITS_IDX) &&
+      dev->type_slot[end] >= 0) {
+    sd.op = OP_FILTER_TYPE;  // T in the types of ?x  <=>  ?x in the instances of T
+    sd.ec = uint32_t(dev->type_slot[end]);
+    sd.seg = -1;
+  }
   sd.out_cols = op == OP_EXPAND ? in->cols + 1 : in->cols;
   return pattern_step(dev, sd, in, out);
 }
@@ -648,7 +695,7 @@ int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols,
   dev->last_rows = 0;
   dev->last_cols = 0;
   Plan plan;
-  compile_query(*dev->st, *q, plan);
+  compile_query(*dev->st, *q, plan, dev->type_slot.data());
   if (rows) *rows = 0;
   if (cols) *cols = 0;
   if (plan.status) return plan.status;
@@ -689,7 +736,7 @@ int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph
   if (plans.size() < nq) plans.resize(nq);
 #pragma omp parallel for schedule(static) if (nq >= 1024)
   for (uint32_t i = 0; i < nq; i++) {
-    compile_query(*dev->st, qs[i], plans[i]);
+    compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
     out->cols[i] = plans[i].cols;
     if (out->v2c)
       for (int32_t v = 0; v < WK_MAX_VARS; v++)
@@ -709,7 +756,7 @@ int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
   std::vector<Plan>& plans = dev->plan_cache;
   if (plans.size() < nq) plans.resize(nq);
 #pragma omp parallel for schedule(static) if (nq >= 1024)
-  for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, qs[i], plans[i]);
+  for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
   std::vector<uint64_t> rows(nq), offs(nq);
   std::vector<int32_t> status(nq);
   static const int kWidths[] = {1, 2, 4, 8};

@@ -18,7 +18,7 @@ namespace {
 enum VStat { KNOWN_VAR = 0, UNKNOWN_VAR = 1, CONST_VAR = 2 };  // core/sparql/query.hpp:56
 }
 
-int compile_query(const Store& st, const wk_query& q, Plan& plan) {
+int compile_query(const Store& st, const wk_query& q, Plan& plan, const int32_t* type_slot) {
   plan.steps.clear();
   plan.cols = 0;
   plan.status = WK_OK;
@@ -96,6 +96,13 @@ int compile_query(const Store& st, const wk_query& q, Plan& plan) {
         sd.sc = col_of(p.subject);
         sd.kend = uint32_t(p.object);
         sd.seg = st.seg_index(0, sd.pid, sd.dir);
+        if (type_slot && sd.pid == TYPE_ID && sd.dir == DIR_OUT && sd.kend < (1u << NBITS_IDX) &&
+            type_slot[sd.kend] >= 0) {
+          // T in the types of ?x  <=>  ?x in the instances of T
+          sd.op = OP_FILTER_TYPE;
+          sd.ec = uint32_t(type_slot[sd.kend]);
+          sd.seg = -1;  // no segment is read
+        }
         break;
       case (KNOWN_VAR << 4) | KNOWN_VAR:  // known_to_known (sparql.hpp:419-479)
         sd.op = OP_FILTER_KNOWN;
@@ -319,6 +326,7 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
   a.ktab = dev->d_ktab;
   a.vtab = dev->d_vtab;
   a.segs = dev->d_segs;
+  a.tbits = dev->d_tbits;
   a.kshift = dev->kshift;
   a.vshift = dev->vshift;
   a.tin = tin;
@@ -440,8 +448,9 @@ int probe_scan(wk_dev* dev, const StepDesc& sd0, const uint32_t* tin, uint64_t R
   const bool direct = sd.seg >= 0 && dev->h_segs[sd.seg].dense && sd.key_mode != KEY_TYPE_INDEX;
   timed_begin(dev, &ev);
   launch_probe(a, direct ? 1 : dev->opts.probe_width, R, dev->stream);
-  // direct key index: 8-B ptr word instead of the 16-B slot
-  timed_end(dev, ev, "k_probe", (direct ? 20 : 28) * Rd);
+  // direct key index: 8-B ptr word instead of the 16-B slot; a type bitmap:
+  // one 4-B word
+  timed_end(dev, ev, "k_probe", (sd.op == OP_FILTER_TYPE ? 16 : direct ? 20 : 28) * Rd);
   timed_begin(dev, &ev);
   launch_scan_tiles(a, dev->stream);
   timed_end(dev, ev, "k_scan_tiles", 16 * ntiles);

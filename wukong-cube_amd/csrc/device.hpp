@@ -42,7 +42,10 @@ struct Plan {
   int status = WK_OK;
 };
 
-int compile_query(const Store& st, const wk_query& q, Plan& plan);
+// type_slot (may be null): for each id below 2^17, the slot of its instance
+// bitmap, or -1; a `?x rdf:type T` known_to_const step of a type with a
+// bitmap becomes OP_FILTER_TYPE.
+int compile_query(const Store& st, const wk_query& q, Plan& plan, const int32_t* type_slot = nullptr);
 
 struct SegSlot {
   uint32_t nkb = 0, nvb = 0;      // blocks needed
@@ -90,6 +93,11 @@ struct wk_dev {
   // arena places)
   std::vector<uint64_t*> dense;
   std::vector<uint8_t> dense_built;
+  // instance bitmaps of the types (OP_FILTER_TYPE): slot per id, device
+  // descriptors by slot, and their word arrays
+  std::vector<int32_t> type_slot;
+  wk::TypeBits* d_tbits = nullptr;
+  std::vector<uint32_t*> tbits_words;
   wk::DevSeg* h_segs = nullptr;   // pinned mirror
   wk::DevSeg* d_segs = nullptr;
   uint32_t* h_ktab = nullptr;     // pinned mirror

@@ -352,6 +352,13 @@ __device__ __forceinline__ ProbeDesc read_probe_desc(const StepDesc* d) {
 
 // ---- probe: per input row, edge count (expand) or keep flag (filters) -----
 
+// OP_FILTER_TYPE: whether row[sc] is an instance of the step's type.
+__device__ __forceinline__ uint32_t type_bit(const StepArgs& a, const ProbeDesc& d, const uint32_t* row) {
+  const TypeBits t = a.tbits[d.ec];
+  const uint32_t i = row[d.sc] - t.vmin;
+  return i < t.nbits ? (t.words[i >> 5] >> (i & 31)) & 1u : 0u;
+}
+
 // One input row: key from the row (or the constant / type index), bucket
 // probe, then the edge count (expand) or the membership flag (filters).  The
 // W lanes of a group call it together with the same row.
@@ -497,6 +504,7 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
       const uint32_t q = __builtin_amdgcn_readfirstlane(q0);
       const ProbeDesc d = read_probe_desc(a.desc + q);
       const bool live = d.op != OP_SKIP && d.seg >= 0;
+      const bool tb = d.op == OP_FILTER_TYPE;
       const DevSeg s = a.segs[live ? d.seg : 0];
       const uint32_t* const tbase = a.tin + a.desc[q].in_off;
       const uint64_t r0 = a.row_start[q];
@@ -510,8 +518,12 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
             const bool has1 = g1 < thi;
             const uint64_t gr1 = has1 ? g1 : g;  // a lone last row probes itself twice
             uint32_t c = 0, eo = 0, c1 = 0, eo1 = 0;
-            if (res)
+            if (tb) {
+              c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
+              c1 = type_bit(a, d, tbase + (gr1 - r0) * d.in_cols);
+            } else if (res) {
               probe_row2<W>(a, d, s, tbase + (g - r0) * d.in_cols, tbase + (gr1 - r0) * d.in_cols, c, eo, c1, eo1);
+            }
             if (gl == 0) {
               a.cnt[g] = c;
               a.eoff[g] = eo;
@@ -529,7 +541,8 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
           const uint64_t g = tlo + uint64_t(pass) * G + grp;
           if (g < thi) {
             uint32_t c = 0, eo = 0;
-            if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
+            if (tb) c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
+            else if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
             if (gl == 0) {
               a.cnt[g] = c;
               a.eoff[g] = eo;
@@ -566,12 +579,16 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
               else h = m;
             }
             const ProbeDesc d = s_pd[l];
-            if (d.op != OP_SKIP && d.seg >= 0)
+            if (d.op == OP_FILTER_TYPE)
+              c = type_bit(a, d, a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols);
+            else if (d.op != OP_SKIP && d.seg >= 0)
               probe_row<W>(a, d, s_sg[l], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
           } else {
             const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
             const ProbeDesc d = read_probe_desc(a.desc + q);
-            if (d.op != OP_SKIP && d.seg >= 0)
+            if (d.op == OP_FILTER_TYPE)
+              c = type_bit(a, d, a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols);
+            else if (d.op != OP_SKIP && d.seg >= 0)
               probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
                            eo);
           }

@@ -19,7 +19,9 @@
 
 namespace wk {
 
-enum : uint32_t { OP_EXPAND = 0, OP_FILTER_KNOWN = 1, OP_FILTER_CONST = 2, OP_SKIP = 3 };
+// OP_FILTER_TYPE: known_to_const `?x rdf:type T` answered from T's instance
+// bitmap (StepDesc.ec holds the bitmap's slot) instead of x's type list
+enum : uint32_t { OP_EXPAND = 0, OP_FILTER_KNOWN = 1, OP_FILTER_CONST = 2, OP_SKIP = 3, OP_FILTER_TYPE = 4 };
 enum : uint32_t { KEY_FROM_ROW = 0, KEY_CONST = 1, KEY_TYPE_INDEX = 2 };
 
 // Segment descriptor as seen by the kernels (device array, one per segment).
@@ -46,6 +48,16 @@ constexpr uint32_t kNoPhys = 0xFFFFFFFFu;
 // not fit the result arena: the query reports WK_E_NOMEM_BUDGET, the others
 // of the step go on.
 constexpr uint64_t kNoSpace = ~uint64_t(0);
+
+// Instances of one type as a bitmap over vids [vmin, vmin + nbits): bit
+// i of words is set when vmin + i has the type, i.e. is in the type index
+// [0|t|IN] (core/store/segment_rdf_dgraph.hpp:951-994), which holds exactly
+// the subjects of the type's rdf:type triples.
+struct TypeBits {
+  const uint32_t* words;
+  uint32_t vmin;
+  uint32_t nbits;
+};
 
 // One query's work in one pattern step.
 struct StepDesc {
@@ -81,6 +93,7 @@ struct StepArgs {
   const uint32_t* ktab;
   const uint32_t* vtab;
   const DevSeg* segs;
+  const TypeBits* tbits;   // type bitmaps by slot (OP_FILTER_TYPE)
   uint32_t kshift, vshift;
   const uint32_t* tin;     // input table buffer
   uint32_t* tout;          // ping-pong output buffer

@@ -193,7 +193,7 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   for (uint32_t i = 1; i < nq; i++)
     if (arrival_s[i] < arrival_s[i - 1]) return WK_E_INVALID_ARG;
   std::vector<Plan> plans(nq);
-  for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, queries[i], plans[i]);
+  for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, queries[i], plans[i], dev->type_slot.data());
   std::vector<QState> qs(nq);
   std::vector<uint64_t> fp(nq);
   for (uint32_t i = 0; i < nq; i++) fp[i] = plans[i].status ? 0 : project_footprint(*dev->st, plans[i]);

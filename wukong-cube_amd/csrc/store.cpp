@@ -379,6 +379,7 @@ int build_store(const uint32_t* pso, uint64_t n_pso, const uint32_t* pos, uint64
         std::vector<uint32_t> v(sizes[k]);
         for (uint32_t j = 0; j < sizes[k]; j++) v[j] = uint32_t(ts[o + j]);
         o += sizes[k];
+        st->type_ids.push_back(his[k]);
         tidx.emplace_back(his[k], std::move(v));
       }
     }

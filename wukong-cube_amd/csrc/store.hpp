@@ -39,6 +39,7 @@ struct Store {
   uint64_t total_keys = 0;
   uint64_t n_out = 0, n_in = 0;
   bool pinned = false;
+  std::vector<uint32_t> type_ids;  // ids with instances [0|t|IN], ascending
 
   ~Store();
   int seg_index(uint32_t index, uint32_t pid, uint32_t dir) const {
