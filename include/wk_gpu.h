@@ -327,7 +327,8 @@ int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* 
 int wk_batch_traffic(wk_dev* dev, int32_t enable, double* alg_bytes, double* edges, uint64_t* batches);
 /* Autotuner (no counterpart in the reference): times the batch of nq
  * queries under every probe width {1,2,4,8} x gather mapping {0,1,4,16,64}
- * and keeps the fastest on the handle.  Outputs may be null. */
+ * (one warm-up run, then the median of three) and keeps the fastest on the
+ * handle.  Outputs may be null. */
 int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, int32_t* probe_width,
                     int32_t* rows_per_wave, double* batch_ms);
 

@@ -221,8 +221,14 @@ def test_batch_arena_exhaustion(univ10_store, univ10_oracle):
     dev.close()
 
 
-def test_gstore_swapping(univ10_store, univ10_oracle):
-    """A gstore budget below the store size evicts and reloads segments."""
+@pytest.mark.parametrize("pinned", [True, False])
+def test_gstore_swapping(univ10, univ10_store, univ10_oracle, pinned):
+    """A gstore budget below the store size evicts and reloads segments, from
+    a pinned store (direct DMA) and from a pageable one (through the pinned
+    staging buffers)."""
+    if not pinned:
+        univ10_store = wg.Store(univ10, pinned=False)
+        assert univ10_store.info()["pinned"] == 0
     info = univ10_store.info()
     total = info["num_buckets"] * 128 + info["num_values"] * 4
     # the largest single segment (type OUT keys, index IN values) still fits;

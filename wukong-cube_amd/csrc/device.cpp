@@ -156,6 +156,37 @@ int evict(wk_dev* dev, int v) {
   return WK_OK;
 }
 
+// A host-to-device copy of store bytes on the copy stream.  From a pinned
+// store it is one DMA; from a pageable one, the bytes go through the ring of
+// pinned staging buffers: host threads copy a piece into the next buffer
+// (once its previous transfer is done) and the DMA goes from there, so
+// filling one buffer overlaps the transfer of the one before.
+int copy_in(wk_dev* dev, void* dst, const void* src, uint64_t n) {
+  if (dev->st->pinned || !dev->stage[0]) {
+    WK_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, dev->copy_stream));
+    return WK_OK;
+  }
+  uint8_t* d = static_cast<uint8_t*>(dst);
+  const uint8_t* s = static_cast<const uint8_t*>(src);
+  while (n) {
+    const int k = dev->stage_next;
+    dev->stage_next = (k + 1) % kStageBufs;
+    if (dev->stage_used[k]) WK_HIP(hipEventSynchronize(dev->stage_ev[k]));
+    const uint64_t m = std::min<uint64_t>(n, dev->stage_bytes);
+    uint8_t* buf = dev->stage[k];
+    constexpr uint64_t kPiece = uint64_t(1) << 20;
+#pragma omp parallel for schedule(static) if (m >= 4 * kPiece)
+    for (uint64_t o = 0; o < m; o += kPiece) std::memcpy(buf + o, s + o, std::min(kPiece, m - o));
+    WK_HIP(hipMemcpyAsync(d, buf, m, hipMemcpyHostToDevice, dev->copy_stream));
+    WK_HIP(hipEventRecord(dev->stage_ev[k], dev->copy_stream));
+    dev->stage_used[k] = 1;
+    d += m;
+    s += m;
+    n -= m;
+  }
+  return WK_OK;
+}
+
 int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
   SegSlot& s = dev->seg[size_t(si)];
   if (s.resident) {
@@ -193,9 +224,9 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
     dev->h_ktab[hs.kb_base + j] = blk;
     const uint64_t b0 = uint64_t(j) * kb;
     const uint64_t n = std::min<uint64_t>(kb, m.total_buckets() - b0);
-    WK_HIP(hipMemcpyAsync(dev->karena + (uint64_t(blk) << dev->kshift) * ASSOCIATIVITY,
-                          dev->st->slots + (m.bucket_start + b0) * ASSOCIATIVITY,
-                          n * ASSOCIATIVITY * sizeof(Slot), hipMemcpyHostToDevice, dev->copy_stream));
+    int rc = copy_in(dev, dev->karena + (uint64_t(blk) << dev->kshift) * ASSOCIATIVITY,
+                     dev->st->slots + (m.bucket_start + b0) * ASSOCIATIVITY, n * ASSOCIATIVITY * sizeof(Slot));
+    if (rc) return rc;
     dev->stats.h2d_bytes += n * ASSOCIATIVITY * sizeof(Slot);
   }
   for (uint32_t j = 0; j < s.nvb; j++) {
@@ -205,8 +236,9 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
     dev->h_vtab[hs.vb_base + j] = blk;
     const uint64_t e0 = uint64_t(j) * vb;
     const uint64_t n = std::min<uint64_t>(vb, m.num_edges - e0);
-    WK_HIP(hipMemcpyAsync(dev->varena + (uint64_t(blk) << dev->vshift), dev->st->values + m.edge_start + e0,
-                          n * sizeof(uint32_t), hipMemcpyHostToDevice, dev->copy_stream));
+    int rc = copy_in(dev, dev->varena + (uint64_t(blk) << dev->vshift), dev->st->values + m.edge_start + e0,
+                     n * sizeof(uint32_t));
+    if (rc) return rc;
     dev->stats.h2d_bytes += n * sizeof(uint32_t);
   }
   if (s.nkb)

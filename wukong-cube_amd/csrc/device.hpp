@@ -55,6 +55,8 @@ struct SegSlot {
   int pins = 0;
 };
 
+constexpr int kStageBufs = 4;  // pinned staging buffers for swaps from pageable memory
+
 struct KernelStat {
   uint64_t launches = 0;
   double ms = 0;
@@ -107,6 +109,12 @@ struct wk_dev {
   uint64_t ktab_n = 0, vtab_n = 0;
   uint64_t tick = 0;
   hipEvent_t copy_done = nullptr, compute_done = nullptr;
+  // pinned staging ring for swaps from a pageable host store
+  uint8_t* stage[wk::kStageBufs] = {};
+  hipEvent_t stage_ev[wk::kStageBufs] = {};
+  uint8_t stage_used[wk::kStageBufs] = {};
+  int stage_next = 0;
+  uint64_t stage_bytes = 0;
   bool copy_pending = false;  // copy-stream work not yet waited for by the compute stream
 
   // tables and scratch
