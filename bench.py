@@ -83,6 +83,9 @@ def parse_args():
     ap.add_argument("--table-budget-gb", type=float, default=48)
     ap.add_argument("--rates", type=str, default="", help="serving offered rates, q/s, comma separated")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--heavy-footprint", type=int, default=0,
+                    help="serving: projected table bytes from which a query runs on the second lane "
+                         "(0 = library default, -1 = no second lane)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
@@ -559,7 +562,8 @@ def main():
             dev.reset_kernel_stats()
             # only the continuous-batching loop keeps its (host-side) iteration timer
             dev.set_profile(mode != "per_query")
-            r = dev.serve(tq[:n], arr, max_inflight=4096, per_query=(mode == "per_query"))
+            r = dev.serve(tq[:n], arr, max_inflight=4096, per_query=(mode == "per_query"),
+                          heavy_footprint=(2**64 - 1) if args.heavy_footprint < 0 else args.heavy_footprint)
             phases = {k[6:]: {"count": v["launches"], "ms": round(v["total_ms"], 3)}
                       for k, v in dev.kernel_stats().items() if k.startswith("serve:")}
             lat = r["latency_us"]

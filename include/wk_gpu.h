@@ -339,6 +339,10 @@ typedef struct {
   uint64_t footprint_budget; /* bytes of projected history tables admitted */
   int32_t per_query;         /* 1: FIFO, one query at a time (per-query baseline) */
   int32_t keep_results;      /* 1: keep each result's rows (for checking) */
+  uint64_t heavy_footprint;  /* queries whose projected history tables reach this many bytes
+                                run as batches on a second lane (its own stream, tables and
+                                scratch, table_budget again), concurrently with the light
+                                queries' iterations; 0 -> 16 MiB, UINT64_MAX -> no second lane */
 } wk_serve_opts;
 
 typedef struct {

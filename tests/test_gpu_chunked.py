@@ -98,15 +98,18 @@ def test_chunked_query_while_serving(small_dev, univ10_oracle, univ10_store):
     qs = [(p, n) for p, n, _ in stream]
     qs.insert(25, (pats, nv))
     arr = np.cumsum(np.random.default_rng(3).exponential(1 / 20000.0, size=len(qs)))
-    r = small_dev.serve(qs, arr, max_inflight=32, keep_results=True)
-    for i, (p, n) in enumerate(qs):
-        want, _, _ = univ10_oracle.query(p, n)
-        if want.size > 65536:
-            assert r["status"][i] == wg.WK_E_NOMEM_BUDGET, i
-            continue
-        assert r["status"][i] == 0, (i, r["status"][i])
-        assert same_rows(small_dev.served_rows(i), want), i
-        assert int(r["checksum"][i]) == (wg.checksum(want) if want.size else 0), i
+    # heavy = 1: every query on the second lane, where the chunked query's
+    # tables overflow and it runs alone, in chunks, on that lane
+    for heavy in (0, 1):
+        r = small_dev.serve(qs, arr, max_inflight=32, keep_results=True, heavy_footprint=heavy)
+        for i, (p, n) in enumerate(qs):
+            want, _, _ = univ10_oracle.query(p, n)
+            if want.size > 65536:
+                assert r["status"][i] == wg.WK_E_NOMEM_BUDGET, i
+                continue
+            assert r["status"][i] == 0, (heavy, i, r["status"][i])
+            assert same_rows(small_dev.served_rows(i), want), (heavy, i)
+            assert int(r["checksum"][i]) == (wg.checksum(want) if want.size else 0), (heavy, i)
 
 
 def test_pattern_entry_points_on_tables_above_the_row_scratch(small_dev, univ10_oracle):

@@ -244,13 +244,16 @@ def test_gstore_swapping(univ10, univ10_store, univ10_oracle, pinned):
     dev.close()
 
 
-def test_serve_results_alone_and_in_mix(univ10_dev, univ10_oracle, univ10_store):
+@pytest.mark.parametrize("heavy", [0, 48 << 10])
+def test_serve_results_alone_and_in_mix(univ10_dev, univ10_oracle, univ10_store, heavy):
     """Continuous batching: each query's rows equal the oracle's (alone) and
-    its checksum equals the per-query path's."""
+    its checksum equals the per-query path's; with heavy = 48 KiB the
+    queries projected above it run as batches on the second lane while the
+    others' iterations go on."""
     stream = Q.template_stream(univ10_store, Q.univ_templates(), 400, seed=3)
-    qs = [(p, nv) for p, nv, _ in stream]
+    qs = [(p, nv) for p, nv, _ in stream] + [Q.lubm_q1(), Q.chain3(), Q.lubm_q7()]
     arr = np.cumsum(np.random.default_rng(1).exponential(1 / 20000.0, size=len(qs)))
-    r = univ10_dev.serve(qs, arr, max_inflight=64, keep_results=True)
+    r = univ10_dev.serve(qs, arr, max_inflight=64, keep_results=True, heavy_footprint=heavy)
     assert (r["status"] == 0).all()
     for i, (pats, nv) in enumerate(qs):
         want, _, _ = univ10_oracle.query(pats, nv)

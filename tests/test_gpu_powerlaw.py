@@ -112,7 +112,8 @@ def test_powerlaw_batch_under_swapping_budget(pl, pl_stream):
     dev.close()
 
 
-def test_powerlaw_serve_light_heavy_mix_swapping(pl, pl_stream):
+@pytest.mark.parametrize("heavy_fp", [0, 1 << 20])
+def test_powerlaw_serve_light_heavy_mix_swapping(pl, pl_stream, heavy_fp):
     """Serving a 95% light (<= 100 rows) / 5% heavy (the largest results) mix
     at a high offered rate under a swapping gstore budget: each served result
     equals the oracle's rows, and its checksum equals the per-query path's
@@ -131,7 +132,7 @@ def test_powerlaw_serve_light_heavy_mix_swapping(pl, pl_stream):
     dev = wg.Device(store, gstore_budget=budget, key_block_shift=10, value_block_shift=14,
                     table_budget=512 << 20)
     arr = np.cumsum(rng.exponential(1 / 50000.0, size=len(mix)))
-    r = dev.serve(mix, arr, max_inflight=256, keep_results=True)
+    r = dev.serve(mix, arr, max_inflight=256, keep_results=True, heavy_footprint=heavy_fp)
     assert (r["status"] == 0).all(), np.unique(r["status"])
     for i, (want, _, _) in enumerate(mix_want):
         assert int(r["rows"][i]) == want.shape[0], i

@@ -22,20 +22,19 @@ uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 void free_dev(wk_dev* d) {
   if (!d) return;
   if (d->stream) (void)hipStreamSynchronize(d->stream);
+  if (d->alt.stream) (void)hipStreamSynchronize(d->alt.stream);
   if (d->copy_stream) (void)hipStreamSynchronize(d->copy_stream);
-  for (auto& g : d->graphs) (void)hipGraphExecDestroy(g.second);
   for (auto& p : d->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
   for (auto e : d->event_pool) (void)hipEventDestroy(e);
-  // tbuf[1] lies inside tbuf[0]'s allocation (one pool of two tables)
-  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs,   d->d_ktab,    d->d_vtab, d->tbuf[0], d->arena,
-                      d->cnt,    d->eoff,   d->pfx,      d->tile_sum,  d->tile_row, d->d_desc, d->d_rs,
-                      d->d_state, d->d_qres, d->d_qagg, d->d_win};
+  free_lane(*d);
+  free_lane(d->alt);
+  void* dev_ptrs[] = {d->karena, d->varena, d->d_segs, d->d_ktab, d->d_vtab};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
-  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab, d->h_desc, d->h_rs, d->h_state, d->h_qres, d->h_win};
+  void* host_ptrs[] = {d->h_segs, d->h_ktab, d->h_vtab};
   for (void* p : host_ptrs)
     if (p) (void)hipHostFree(p);
   for (uint64_t* p : d->dense)
@@ -48,10 +47,56 @@ void free_dev(wk_dev* d) {
   }
   if (d->copy_done) (void)hipEventDestroy(d->copy_done);
   if (d->compute_done) (void)hipEventDestroy(d->compute_done);
-  if (d->stream) (void)hipStreamDestroy(d->stream);
   if (d->copy_stream) (void)hipStreamDestroy(d->copy_stream);
   delete d;
 }
+
+}  // namespace
+
+namespace wk {
+
+void free_lane(LaneState& l) {
+  for (auto& g : l.graphs) (void)hipGraphExecDestroy(g.second);
+  l.graphs.clear();
+  // tbuf[1] lies inside tbuf[0]'s allocation (one pool of two tables)
+  void* dev_ptrs[] = {l.tbuf[0], l.arena,  l.cnt,  l.eoff,   l.pfx,  l.tile_sum, l.tile_row,
+                      l.d_desc,  l.d_rs,   l.d_state, l.d_qres, l.d_qagg, l.d_ck,     l.d_win};
+  for (void* p : dev_ptrs)
+    if (p) (void)hipFree(p);
+  void* host_ptrs[] = {l.h_desc, l.h_rs, l.h_state, l.h_qres, l.h_ck, l.h_win};
+  for (void* p : host_ptrs)
+    if (p) (void)hipHostFree(p);
+  if (l.stream) (void)hipStreamDestroy(l.stream);
+  l = LaneState{};
+}
+
+int alloc_lane(wk_dev* d, uint64_t budget) {
+  // history tables + scratch: 2 ping-pong tables + arena (4 B each per
+  // element) and per-row count/offset/prefix (16 B), rows <= elements
+  const uint64_t tb = budget ? budget : (uint64_t(8) << 30);
+  d->tcap = std::max<uint64_t>(tb / 28, 1 << 16);
+  d->arena_cap = d->tcap;
+  d->rcap = d->tcap;
+  d->tile_cap = d->rcap / kTile + 2;
+  d->otile_cap = (d->tcap + d->arena_cap) / kOutTile + 2;
+  // the two ping-pong tables are one pool, which the per-query path uses as
+  // a stack of intermediate tables
+  WK_HIP(hipMalloc(&d->tbuf[0], 2 * d->tcap * sizeof(uint32_t)));
+  d->tbuf[1] = d->tbuf[0] + d->tcap;
+  WK_HIP(hipMalloc(&d->d_win, size_t(kWinGroup) * 3 * sizeof(uint64_t)));
+  WK_HIP(hipHostMalloc(&d->h_win, size_t(kWinGroup) * 3 * sizeof(uint64_t), hipHostMallocDefault));
+  WK_HIP(hipMalloc(&d->arena, d->arena_cap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->cnt, d->rcap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->eoff, d->rcap * sizeof(uint32_t)));
+  WK_HIP(hipMalloc(&d->pfx, (d->rcap + 1) * sizeof(uint64_t)));
+  WK_HIP(hipMalloc(&d->tile_sum, d->tile_cap * sizeof(uint64_t)));
+  WK_HIP(hipMalloc(&d->tile_row, (d->otile_cap + 1) * sizeof(uint64_t)));
+  return reserve_steps(d, 8, 64);
+}
+
+}  // namespace wk
+
+namespace {
 
 // Direct key indexes for the normal segments whose key vids are dense enough
 // (at least one key per 16 vids of their range: 8 B per vid, so at most
@@ -228,27 +273,7 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   for (uint64_t i = nk; i-- > 0;) d->free_k.push_back(uint32_t(i));
   for (uint64_t i = nv; i-- > 0;) d->free_v.push_back(uint32_t(i));
 
-  // history tables + scratch: 2 ping-pong tables + arena (4 B each per
-  // element) and per-row count/offset/prefix (16 B), rows <= elements
-  const uint64_t tb = o->table_budget ? o->table_budget : (uint64_t(8) << 30);
-  d->tcap = std::max<uint64_t>(tb / 28, 1 << 16);
-  d->arena_cap = d->tcap;
-  d->rcap = d->tcap;
-  d->tile_cap = d->rcap / kTile + 2;
-  d->otile_cap = (d->tcap + d->arena_cap) / kOutTile + 2;
-  // the two ping-pong tables are one pool, which the per-query path uses as
-  // a stack of intermediate tables
-  WK_HIP(hipMalloc(&d->tbuf[0], 2 * d->tcap * sizeof(uint32_t)));
-  d->tbuf[1] = d->tbuf[0] + d->tcap;
-  WK_HIP(hipMalloc(&d->d_win, size_t(kWinGroup) * 3 * sizeof(uint64_t)));
-  WK_HIP(hipHostMalloc(&d->h_win, size_t(kWinGroup) * 3 * sizeof(uint64_t), hipHostMallocDefault));
-  WK_HIP(hipMalloc(&d->arena, d->arena_cap * sizeof(uint32_t)));
-  WK_HIP(hipMalloc(&d->cnt, d->rcap * sizeof(uint32_t)));
-  WK_HIP(hipMalloc(&d->eoff, d->rcap * sizeof(uint32_t)));
-  WK_HIP(hipMalloc(&d->pfx, (d->rcap + 1) * sizeof(uint64_t)));
-  WK_HIP(hipMalloc(&d->tile_sum, d->tile_cap * sizeof(uint64_t)));
-  WK_HIP(hipMalloc(&d->tile_row, (d->otile_cap + 1) * sizeof(uint64_t)));
-  int rc = reserve_steps(d, 8, 64);
+  int rc = alloc_lane(d, o->table_budget);
   if (rc) return rc;
   rc = alloc_dense(d);
   if (rc) return rc;
@@ -422,8 +447,11 @@ int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
     return WK_E_INVALID_ARG;
   if (dev->opts.probe_width != probe_width || dev->opts.rows_per_wave != rows_per_wave) {
     (void)hipStreamSynchronize(dev->stream);
+    if (dev->alt.stream) (void)hipStreamSynchronize(dev->alt.stream);
     for (auto& g : dev->graphs) (void)hipGraphExecDestroy(g.second);
     dev->graphs.clear();
+    for (auto& g : dev->alt.graphs) (void)hipGraphExecDestroy(g.second);
+    dev->alt.graphs.clear();
   }
   dev->opts.probe_width = probe_width;
   dev->opts.rows_per_wave = rows_per_wave;

@@ -149,7 +149,7 @@ int evict(wk_dev* dev, int v) {
   dev->h_segs[v].resident = 0;
   dev->h_segs[v].kphys = kNoPhys;
   dev->h_segs[v].vphys = kNoPhys;
-  dev->copy_pending = true;
+  dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
   WK_HIP(hipMemcpyAsync(dev->d_segs + v, dev->h_segs + v, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
   dev->stats.segments_evicted++;
@@ -211,7 +211,7 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
     if (rc) return rc;
   }
   // blocks being reused may still be read by queued kernels
-  dev->copy_pending = true;
+  dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
   WK_HIP(hipEventRecord(dev->compute_done, dev->stream));
   WK_HIP(hipStreamWaitEvent(dev->copy_stream, dev->compute_done, 0));
   const SegMeta& m = dev->st->segs[size_t(si)];
@@ -325,6 +325,10 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq) {
   if (dev->d_qres) (void)hipFree(dev->d_qres);
   if (dev->h_qres) (void)hipHostFree(dev->h_qres);
   if (dev->d_qagg) (void)hipFree(dev->d_qagg);
+  if (dev->d_ck) (void)hipFree(dev->d_ck);
+  if (dev->h_ck) (void)hipHostFree(dev->h_ck);
+  dev->d_ck = nullptr;
+  dev->h_ck = nullptr;
   dev->d_qagg = nullptr;
   dev->d_qres = nullptr;
   dev->h_qres = nullptr;
@@ -345,6 +349,8 @@ int reserve_steps(wk_dev* dev, uint32_t steps, uint32_t nq) {
   WK_HIP(hipMalloc(&dev->d_qres, size_t(qc) * 2 * sizeof(uint64_t)));
   WK_HIP(hipHostMalloc(&dev->h_qres, size_t(qc) * 2 * sizeof(uint64_t), hipHostMallocDefault));
   WK_HIP(hipMalloc(&dev->d_qagg, (size_t(qc) + kBlock - 1) / kBlock * 3 * sizeof(uint64_t)));
+  WK_HIP(hipMalloc(&dev->d_ck, size_t(qc) * sizeof(uint64_t)));
+  WK_HIP(hipHostMalloc(&dev->h_ck, size_t(qc) * sizeof(uint64_t), hipHostMallocDefault));
   dev->steps_cap = sc;
   dev->nq_cap = qc;
   return WK_OK;
@@ -432,8 +438,8 @@ double host_phase(wk_dev* dev, const char* name, double t0) {
 
 int flush_timings(wk_dev* dev) {
   if (dev->pending.empty()) return WK_OK;
-  WK_HIP(hipStreamSynchronize(dev->stream));
   for (auto& p : dev->pending) {
+    WK_HIP(hipEventSynchronize(p.b));  // on whichever lane's stream it was taken
     float ms = 0;
     (void)hipEventElapsedTime(&ms, p.a, p.b);
     KernelStat& k = dev->kstats[p.name];

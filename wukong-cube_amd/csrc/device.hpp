@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -77,11 +78,62 @@ struct wk_table {
   uint32_t cols = 0;
 };
 
-struct wk_dev {
+namespace wk {
+
+// What one lane of work owns: a stream, its history tables and result
+// arena, per-row scratch and step descriptors.  The device has a main lane
+// and, for serving, a second one that runs heavy queries concurrently on its
+// own stream; the gstore, its cache and the direct indexes are shared.
+struct LaneState {
+  hipStream_t stream = nullptr;
+  bool copy_pending = false;  // copy-stream work not yet waited for by this lane's stream
+
+  // tables and scratch
+  uint32_t* tbuf[2] = {nullptr, nullptr};
+  uint64_t tcap = 0;
+  uint32_t* arena = nullptr;
+  uint64_t arena_cap = 0;
+  uint32_t* cnt = nullptr;
+  uint32_t* eoff = nullptr;
+  uint64_t* pfx = nullptr;
+  uint64_t rcap = 0;
+  uint64_t* tile_sum = nullptr;
+  uint64_t tile_cap = 0;
+  uint64_t* tile_row = nullptr;
+  uint64_t otile_cap = 0;
+
+  // step descriptors: device arrays sized for nsteps_cap x nq_cap
+  StepDesc* d_desc = nullptr;
+  uint64_t* d_rs = nullptr;
+  StepState* d_state = nullptr;
+  StepDesc* h_desc = nullptr;   // pinned staging
+  uint64_t* h_rs = nullptr;
+  StepState* h_state = nullptr;
+  uint64_t* d_qres = nullptr;       // [nq_cap][2] final rows / arena offset per query
+  uint64_t* h_qres = nullptr;       // pinned staging
+  uint64_t* d_qagg = nullptr;       // [ceil(nq_cap / 256)][3] query-scan block sums
+  uint64_t* d_ck = nullptr;         // [nq_cap] row-hash sums of final tables
+  uint64_t* h_ck = nullptr;         // pinned staging
+  uint32_t nq_cap = 0, steps_cap = 0;
+
+  // hipGraph cache for batches: key (steps, nq)
+  std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs;
+
+  // chunk boundaries of a step run in output windows: [kWinGroup][3]
+  uint64_t* d_win = nullptr;
+  uint64_t* h_win = nullptr;  // pinned mirror
+};
+
+}  // namespace wk
+
+struct wk_dev : wk::LaneState {
   const wk::Store* st = nullptr;
   int device = 0;
-  hipStream_t stream = nullptr, copy_stream = nullptr;
+  hipStream_t copy_stream = nullptr;
   wk_dev_opts opts{};
+  // the other lane: swapped with the device's own fields while its work is
+  // issued (the serving loop's heavy queries)
+  wk::LaneState alt;
 
   // gstore cache
   uint32_t kshift = 14, vshift = 19;
@@ -115,43 +167,9 @@ struct wk_dev {
   uint8_t stage_used[wk::kStageBufs] = {};
   int stage_next = 0;
   uint64_t stage_bytes = 0;
-  bool copy_pending = false;  // copy-stream work not yet waited for by the compute stream
-
-  // tables and scratch
-  uint32_t* tbuf[2] = {nullptr, nullptr};
-  uint64_t tcap = 0;
-  uint32_t* arena = nullptr;
-  uint64_t arena_cap = 0;
-  uint32_t* cnt = nullptr;
-  uint32_t* eoff = nullptr;
-  uint64_t* pfx = nullptr;
-  uint64_t rcap = 0;
-  uint64_t* tile_sum = nullptr;
-  uint64_t tile_cap = 0;
-  uint64_t* tile_row = nullptr;
-  uint64_t otile_cap = 0;
-
-  // step descriptors: device arrays sized for nsteps_cap x nq_cap
-  wk::StepDesc* d_desc = nullptr;
-  uint64_t* d_rs = nullptr;
-  wk::StepState* d_state = nullptr;
-  wk::StepDesc* h_desc = nullptr;   // pinned staging
-  uint64_t* h_rs = nullptr;
-  wk::StepState* h_state = nullptr;
-  uint64_t* d_qres = nullptr;       // [nq_cap][2] final rows / arena offset per query
-  uint64_t* h_qres = nullptr;       // pinned staging
-  uint64_t* d_qagg = nullptr;       // [ceil(nq_cap / 256)][3] query-scan block sums
-  uint32_t nq_cap = 0, steps_cap = 0;
 
   // compiled plans of the last batch (storage reused by the next)
   std::vector<wk::Plan> plan_cache;
-
-  // hipGraph cache for batches: key (steps, nq)
-  std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs;
-
-  // chunk boundaries of a step run in output windows: [kWinGroup][3]
-  uint64_t* d_win = nullptr;
-  uint64_t* h_win = nullptr;  // pinned mirror
 
   // last per-query result (in the result arena)
   const uint32_t* last_ptr = nullptr;
@@ -178,6 +196,15 @@ struct wk_dev {
 };
 
 namespace wk {
+
+// Exchange the device's own lane fields with dev->alt (the lanes' streams,
+// tables, scratch and descriptors).
+inline void swap_lanes(wk_dev* dev) { std::swap(static_cast<LaneState&>(*dev), dev->alt); }
+
+// Tables, scratch and step descriptors of the lane now in the device's own
+// fields, for a table budget of `budget` bytes (0: 8 GiB); its stream exists.
+int alloc_lane(wk_dev* dev, uint64_t budget);
+void free_lane(wk::LaneState& l);
 
 // Make `segs` resident (pinned against eviction while in use by the launch
 // sequence that follows); the compute stream waits for their copies.

@@ -793,7 +793,7 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
     StepDesc& d = a.desc[q];
     const uint64_t el = rows * oc;
     d.out_rows = rows;
-    if (a.ck) a.ck[q] = 0;  // the gather adds the final rows' hashes
+    if (a.ck && fin) a.ck[q] = 0;  // the gather adds the final rows' hashes
     if (a.next) {
       a.next[q].in_rows = fin ? 0 : rows;
       a.next[q].in_off = xt;
@@ -947,7 +947,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
   const uint64_t xn = block_excl_scan<kBlock>(fin ? 0 : rows, nullptr) + v[2];
   if (q < a.nq) {
     StepDesc& d = a.desc[q];
-    if (a.ck) a.ck[q] = 0;  // the gather adds the final rows' hashes
+    if (a.ck && fin) a.ck[q] = 0;  // the gather adds the final rows' hashes
     // a final table that does not fit the arena is dropped alone
     const bool fits = !fin || xa + el <= a.arena_cap;
     d.out_rows = fits ? rows : 0;

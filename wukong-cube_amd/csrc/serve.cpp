@@ -17,6 +17,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+constexpr size_t kHeavyBatch = 8;  // heavy queries launched together on the second lane
+
 struct QState {
   uint32_t k = 0;      // next step
   uint64_t rows = 0;   // rows of the current table
@@ -182,6 +184,119 @@ struct Server {
   }
 };
 
+// Heavy queries on the second lane: a batch of them runs there, all its
+// pattern steps launched at once on that lane's stream, while the light
+// queries' iterations go on on the main lane; the scheduler polls the batch's
+// completion event.  Each query's final table and its row-hash sum (summed by
+// the gather as it writes the rows) stay on the lane until finished.
+struct HeavyLane {
+  wk_dev* dev;
+  const std::vector<Plan>& plans;
+  bool busy = false;
+  std::vector<uint32_t> q;   // the batch's queries
+  std::vector<int> segs;     // its segments, pinned while it runs
+  uint32_t S = 0;
+  hipEvent_t done = nullptr;
+
+  HeavyLane(wk_dev* d, const std::vector<Plan>& p) : dev(d), plans(p) {}
+  ~HeavyLane() {
+    if (done) (void)hipEventDestroy(done);
+  }
+
+  // The second lane's stream, tables and scratch, made once per device.
+  int ready() {
+    if (!done) WK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    if (dev->alt.stream) return WK_OK;
+    swap_lanes(dev);
+    int rc = WK_OK;
+    if (hip_failed(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking))) rc = WK_E_HIP;
+    if (!rc) rc = alloc_lane(dev, dev->opts.table_budget);
+    if (rc) free_lane(*dev);
+    swap_lanes(dev);
+    return rc;
+  }
+
+  // Launch the batch `b` on the second lane.  Returns WK_E_NOMEM_BUDGET when
+  // its segments do not fit the gstore beside the pinned ones (nothing is
+  // launched then).
+  int launch(const std::vector<uint32_t>& b) {
+    swap_lanes(dev);
+    const int rc = launch_swapped(b);
+    swap_lanes(dev);
+    return rc;
+  }
+
+  int launch_swapped(const std::vector<uint32_t>& b) {
+    const uint32_t n = uint32_t(b.size());
+    S = 0;
+    for (uint32_t i : b) S = std::max<uint32_t>(S, uint32_t(plans[i].steps.size()));
+    int rc = reserve_steps(dev, S, n);
+    if (rc) return rc;
+    const size_t qc = dev->nq_cap;
+    std::vector<uint8_t> listed(dev->seg.size(), 0);
+    segs.clear();
+    StepDesc skip{};
+    skip.op = OP_SKIP;
+    skip.seg = -1;
+    for (uint32_t i = 0; i < n; i++) {
+      const Plan& p = plans[b[i]];
+      const uint32_t ns = uint32_t(p.steps.size());
+      dev->h_rs[i] = i;
+      for (uint32_t k = 0; k < S; k++) {
+        StepDesc* d = dev->h_desc + k * qc + i;
+        if (k < ns) {
+          *d = p.steps[k];
+          d->final_step = (k + 1 == ns) ? 1 : 0;
+          d->in_rows = k == 0 ? 1 : 0;
+          d->in_off = 0;
+          d->out_rows = d->out_off = 0;
+          if (d->seg >= 0 && !listed[size_t(d->seg)]) {
+            listed[size_t(d->seg)] = 1;
+            segs.push_back(d->seg);
+          }
+        } else {
+          *d = skip;
+        }
+      }
+    }
+    dev->h_rs[n] = n;
+    rc = ensure_segments(dev, segs);
+    if (rc) return rc;
+    for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
+    if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, ((S - 1) * qc + n) * sizeof(StepDesc),
+                                  hipMemcpyHostToDevice, dev->stream)) ||
+        hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                  dev->stream)) ||
+        hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState), hipMemcpyHostToDevice,
+                                  dev->stream))) {
+      unpin_segments(dev, segs);
+      return WK_E_HIP;
+    }
+    for (uint32_t k = 0; k < S; k++) {
+      StepArgs a = make_args(dev, k, n, dev->tbuf[(k + 1) & 1], dev->tbuf[k & 1], dev->tcap, k + 1 < S);
+      a.qres = dev->d_qres;
+      a.ck = reinterpret_cast<unsigned long long*>(dev->d_ck);
+      launch_probe(a, dev->opts.probe_width, 0, dev->stream);
+      launch_expand_tail(a, dev->opts.rows_per_wave, dev->stream);
+    }
+    if (hip_failed(hipMemcpyAsync(dev->h_qres, dev->d_qres, size_t(n) * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  dev->stream)) ||
+        hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost,
+                                  dev->stream)) ||
+        hip_failed(hipMemcpyAsync(dev->h_ck, dev->d_ck, size_t(n) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  dev->stream)) ||
+        hip_failed(hipEventRecord(done, dev->stream))) {
+      unpin_segments(dev, segs);
+      return WK_E_HIP;
+    }
+    q = b;
+    busy = true;
+    return WK_OK;
+  }
+
+  bool finished() const { return busy && hipEventQuery(done) == hipSuccess; }
+};
+
 }  // namespace
 }  // namespace wk
 
@@ -198,6 +313,9 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   std::vector<uint64_t> fp(nq);
   for (uint32_t i = 0; i < nq; i++) fp[i] = plans[i].status ? 0 : project_footprint(*dev->st, plans[i]);
   const uint32_t max_inflight = opts->per_query ? 1 : std::max<uint32_t>(1, opts->max_inflight);
+  // queries projected above heavy_footprint run on the second lane
+  const uint64_t heavy_fp = opts->heavy_footprint ? opts->heavy_footprint : (uint64_t(16) << 20);
+  const bool use_heavy = !opts->per_query && heavy_fp != ~uint64_t(0);
   const uint64_t budget = opts->footprint_budget ? opts->footprint_budget : dev->tcap * 4;
   if (opts->keep_results) {
     dev->served_rows.assign(nq, {});
@@ -237,6 +355,12 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   std::vector<int> failed;
   CkDesc* d_ck = nullptr;
   uint64_t* d_sum = nullptr;
+  HeavyLane heavy(dev, plans);
+  std::vector<uint32_t> heavy_queue;
+  if (use_heavy) {
+    int hr = heavy.ready();
+    if (hr) return hr;
+  }
   // one query by itself, pattern by pattern with a host sync per pattern,
   // its result into the (then unused) result arena
   auto run_alone = [&](uint32_t q) -> int {
@@ -262,8 +386,41 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   };
   WK_HIP(hipMalloc(&d_ck, sizeof(CkDesc)));
   WK_HIP(hipMalloc(&d_sum, sizeof(uint64_t)));
+  // a completed heavy batch: each query's rows and checksum, or, if its
+  // tables overflowed the lane's or its final table did not fit the lane's
+  // arena, the query alone on the chunking path (on that lane)
+  auto finish_heavy = [&]() -> int {
+    swap_lanes(dev);
+    unpin_segments(dev, heavy.segs);
+    heavy.busy = false;
+    bool overflow = false;
+    for (uint32_t k = 0; k < heavy.S; k++) overflow |= dev->h_state[k].status != 0;
+    const double t = now_s();
+    int r = WK_OK;
+    std::vector<uint32_t> alone;
+    for (uint32_t i = 0; i < heavy.q.size() && r == WK_OK; i++) {
+      const uint32_t q = heavy.q[i];
+      const uint64_t off = dev->h_qres[2 * size_t(i) + 1];
+      if (overflow || off == kNoSpace) {
+        alone.push_back(q);
+        continue;
+      }
+      const uint64_t rows = dev->h_qres[2 * size_t(i)];
+      r = keep(q, dev->arena + off, rows, plans[q].cols);
+      finish(q, t, rows, WK_OK, rows ? dev->h_ck[i] : 0);
+      ndone++;
+    }
+    for (uint32_t q : alone)
+      if (r == WK_OK) r = run_alone(q);
+    swap_lanes(dev);
+    return r;
+  };
   int rc = WK_OK;
   while (ndone < nq && rc == WK_OK) {
+    if (heavy.finished()) {
+      rc = finish_heavy();
+      if (rc) break;
+    }
     double now = now_s();
     if (!retry.empty() && inflight.empty()) {
       const uint32_t q = retry.front();
@@ -282,6 +439,13 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
         next++;
         continue;
       }
+      if (use_heavy && fp[q] >= heavy_fp) {
+        qs[q] = QState{};
+        qs[q].admit = now;
+        heavy_queue.push_back(q);
+        next++;
+        continue;
+      }
       if (!inflight.empty() && fp_used + fp[q] > budget) break;
       qs[q] = QState{};
       qs[q].admit = now;
@@ -289,11 +453,29 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
       inflight.push_back(q);
       next++;
     }
-    if (inflight.empty()) {
-      if (next < nq) {
-        const double wait = arrival_s[next] - now_s();
-        if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+    if (!heavy.busy && !heavy_queue.empty()) {
+      // the next heavy batch: the oldest waiting heavy queries
+      const size_t nb = std::min<size_t>(heavy_queue.size(), kHeavyBatch);
+      std::vector<uint32_t> b(heavy_queue.begin(), heavy_queue.begin() + nb);
+      const int lr = heavy.launch(b);
+      if (dev->opts.profile) {
+        dev->kstats["serve:heavy_batches"].launches++;
+        dev->kstats["serve:heavy_queries"].launches += nb;
       }
+      if (lr == WK_OK || lr == WK_E_NOMEM_BUDGET || lr == WK_E_SEG_TOO_LARGE) {
+        heavy_queue.erase(heavy_queue.begin(), heavy_queue.begin() + nb);
+        // segments that do not fit beside the pinned ones: alone, later
+        if (lr) retry.insert(retry.end(), b.begin(), b.end());
+      } else {
+        rc = lr;
+        break;
+      }
+    }
+    if (inflight.empty()) {
+      double wait = next < nq ? arrival_s[next] - now_s() : 1.0;
+      if (heavy.busy) wait = std::min(wait, 50e-6);  // poll the heavy lane
+      if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+      else if (heavy.busy && wait > 0) std::this_thread::yield();
       continue;
     }
     batches++;
@@ -341,6 +523,12 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     }
     inflight.swap(still);
     cur ^= 1;
+  }
+  if (heavy.busy) {
+    // stopped by an error: let the heavy batch finish before its segments
+    // are released
+    (void)hipEventSynchronize(heavy.done);
+    unpin_segments(dev, heavy.segs);
   }
   if (d_ck) (void)hipFree(d_ck);
   if (d_sum) (void)hipFree(d_sum);

@@ -95,7 +95,7 @@ class BatchResult(C.Structure):
 
 class ServeOpts(C.Structure):
     _fields_ = [("max_inflight", C.c_uint32), ("footprint_budget", C.c_uint64),
-                ("per_query", C.c_int32), ("keep_results", C.c_int32)]
+                ("per_query", C.c_int32), ("keep_results", C.c_int32), ("heavy_footprint", C.c_uint64)]
 
 
 class ServeResult(C.Structure):
@@ -695,7 +695,7 @@ class Device:
         return pw.value, rpw.value, ms.value
 
     def serve(self, queries, arrivals, max_inflight=4096, footprint_budget=0, per_query=False,
-              keep_results=False):
+              keep_results=False, heavy_footprint=0):
         n = len(queries)
         qs = (Query * max(1, n))()
         keep = []
@@ -712,7 +712,8 @@ class Device:
         res = ServeResult(lat.ctypes.data_as(C.POINTER(C.c_double)), start.ctypes.data_as(C.POINTER(C.c_double)),
                           rows.ctypes.data_as(U64P), status.ctypes.data_as(C.POINTER(C.c_int32)),
                           ck.ctypes.data_as(U64P), 0.0, 0)
-        o = ServeOpts(max_inflight, footprint_budget, 1 if per_query else 0, 1 if keep_results else 0)
+        o = ServeOpts(max_inflight, footprint_budget, 1 if per_query else 0, 1 if keep_results else 0,
+                      heavy_footprint)
         _check(lib().wk_serve_replay(self._h, qs, arr.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(o),
                                      C.byref(res)), "wk_serve_replay")
         return {"latency_us": lat, "start_us": start, "rows": rows, "status": status, "checksum": ck,
