@@ -342,7 +342,10 @@ typedef struct {
   uint64_t heavy_footprint;  /* queries whose projected history tables reach this many bytes
                                 run as batches on a second lane (its own stream, tables and
                                 scratch, table_budget again), concurrently with the light
-                                queries' iterations; 0 -> 16 MiB, UINT64_MAX -> no second lane */
+                                queries' iterations: projected at admission, and again from
+                                the table each main-lane step made (the query then continues
+                                on the lane from a copy of that table); 0 -> 1 MiB,
+                                UINT64_MAX -> no second lane */
 } wk_serve_opts;
 
 typedef struct {

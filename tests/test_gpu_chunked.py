@@ -99,8 +99,9 @@ def test_chunked_query_while_serving(small_dev, univ10_oracle, univ10_store):
     qs.insert(25, (pats, nv))
     arr = np.cumsum(np.random.default_rng(3).exponential(1 / 20000.0, size=len(qs)))
     # heavy = 1: every query on the second lane, where the chunked query's
-    # tables overflow and it runs alone, in chunks, on that lane
-    for heavy in (0, 1):
+    # tables overflow and it runs alone, in chunks, on that lane; heavy =
+    # 4096: queries also move there after a step, with the table it made
+    for heavy in (0, 1, 4096):
         r = small_dev.serve(qs, arr, max_inflight=32, keep_results=True, heavy_footprint=heavy)
         for i, (p, n) in enumerate(qs):
             want, _, _ = univ10_oracle.query(p, n)

@@ -112,12 +112,14 @@ def test_powerlaw_batch_under_swapping_budget(pl, pl_stream):
     dev.close()
 
 
-@pytest.mark.parametrize("heavy_fp", [0, 1 << 20])
+@pytest.mark.parametrize("heavy_fp", [0, 1 << 20, 64 << 10])
 def test_powerlaw_serve_light_heavy_mix_swapping(pl, pl_stream, heavy_fp):
     """Serving a 95% light (<= 100 rows) / 5% heavy (the largest results) mix
     at a high offered rate under a swapping gstore budget: each served result
     equals the oracle's rows, and its checksum equals the per-query path's
-    (the same query alone)."""
+    (the same query alone).  With the lower heavy thresholds, queries whose
+    tables grow past the threshold continue on the second lane from the
+    table made on the main lane."""
     _, _, store = pl
     qs, _, wants = pl_stream
     rows = np.array([w[0].shape[0] for w in wants])
@@ -132,8 +134,13 @@ def test_powerlaw_serve_light_heavy_mix_swapping(pl, pl_stream, heavy_fp):
     dev = wg.Device(store, gstore_budget=budget, key_block_shift=10, value_block_shift=14,
                     table_budget=512 << 20)
     arr = np.cumsum(rng.exponential(1 / 50000.0, size=len(mix)))
+    dev.set_profile(True)
     r = dev.serve(mix, arr, max_inflight=256, keep_results=True, heavy_footprint=heavy_fp)
+    ks = dev.kernel_stats()
+    dev.set_profile(False)
     assert (r["status"] == 0).all(), np.unique(r["status"])
+    if heavy_fp:
+        assert ks["serve:heavy_continued"]["launches"] > 0
     for i, (want, _, _) in enumerate(mix_want):
         assert int(r["rows"][i]) == want.shape[0], i
         assert same_rows(dev.served_rows(i), want), i

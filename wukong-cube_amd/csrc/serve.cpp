@@ -27,11 +27,15 @@ struct QState {
   uint32_t retries = 0;  // restarts after its step did not fit beside others
 };
 
-// Projected history-table bytes of a query: exact first-pattern cardinality
-// from the host store, then the average degree of each expanded segment.
-uint64_t project_footprint(const Store& st, const Plan& p) {
-  double rows = 1, best = 0;
-  for (size_t k = 0; k < p.steps.size(); k++) {
+// Projected history-table bytes of a query from step k on, given the rows of
+// its table before that step: the exact cardinality of a constant start from
+// the host store, then the average degree of each expanded segment.  At
+// admission (k = 0, one virtual start row) this underestimates queries that
+// walk through high-degree vertices; the measured table after each step
+// corrects it.
+uint64_t project_footprint(const Store& st, const Plan& p, size_t k0 = 0, uint64_t rows0 = 1) {
+  double rows = double(rows0), best = 0;
+  for (size_t k = k0; k < p.steps.size(); k++) {
     const StepDesc& sd = p.steps[k];
     const double in = rows;
     if (sd.op == OP_EXPAND) {
@@ -187,9 +191,19 @@ struct Server {
 // Heavy queries on the second lane: a batch of them runs there, all its
 // pattern steps launched at once on that lane's stream, while the light
 // queries' iterations go on on the main lane; the scheduler polls the batch's
-// completion event.  Each query's final table and its row-hash sum (summed by
-// the gather as it writes the rows) stay on the lane until finished.
+// completion event.  A query enters the lane at its first step (projected
+// heavy at admission) or, once a main-lane step has made a table whose
+// projection is heavy, at its next step, continuing from a copy of that
+// table.  Each query's final table and its row-hash sum (summed by the
+// gather as it writes the rows) stay on the lane until finished.
 struct HeavyLane {
+  // A query of a batch: its first step on the lane and, when k0 > 0, the rows
+  // and element offset of its table in the main lane's current input buffer.
+  struct Entry {
+    uint32_t q, k0;
+    uint64_t rows, off;
+  };
+
   wk_dev* dev;
   const std::vector<Plan>& plans;
   bool busy = false;
@@ -197,15 +211,18 @@ struct HeavyLane {
   std::vector<int> segs;     // its segments, pinned while it runs
   uint32_t S = 0;
   hipEvent_t done = nullptr;
+  hipEvent_t seeded = nullptr;
 
   HeavyLane(wk_dev* d, const std::vector<Plan>& p) : dev(d), plans(p) {}
   ~HeavyLane() {
     if (done) (void)hipEventDestroy(done);
+    if (seeded) (void)hipEventDestroy(seeded);
   }
 
   // The second lane's stream, tables and scratch, made once per device.
   int ready() {
     if (!done) WK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    if (!seeded) WK_HIP(hipEventCreateWithFlags(&seeded, hipEventDisableTiming));
     if (dev->alt.stream) return WK_OK;
     swap_lanes(dev);
     int rc = WK_OK;
@@ -216,39 +233,54 @@ struct HeavyLane {
     return rc;
   }
 
-  // Launch the batch `b` on the second lane.  Returns WK_E_NOMEM_BUDGET when
-  // its segments do not fit the gstore beside the pinned ones (nothing is
-  // launched then).
-  int launch(const std::vector<uint32_t>& b) {
+  // Elements of the lane's first input table (the continued tables).
+  uint64_t seed_cap() const { return dev->alt.tcap; }
+
+  // Launch the batch `b` on the second lane; `main_tin` is the main lane's
+  // current input buffer, which holds the continued tables.  Returns
+  // WK_E_NOMEM_BUDGET when its segments do not fit the gstore beside the
+  // pinned ones (nothing is launched then).
+  int launch(const std::vector<Entry>& b, const uint32_t* main_tin) {
+    hipStream_t main = dev->stream;
     swap_lanes(dev);
-    const int rc = launch_swapped(b);
+    const int rc = launch_swapped(b, main_tin, main);
     swap_lanes(dev);
     return rc;
   }
 
-  int launch_swapped(const std::vector<uint32_t>& b) {
+  int launch_swapped(const std::vector<Entry>& b, const uint32_t* main_tin, hipStream_t main) {
     const uint32_t n = uint32_t(b.size());
     S = 0;
-    for (uint32_t i : b) S = std::max<uint32_t>(S, uint32_t(plans[i].steps.size()));
+    for (const Entry& e : b) S = std::max<uint32_t>(S, uint32_t(plans[e.q].steps.size()) - e.k0);
     int rc = reserve_steps(dev, S, n);
     if (rc) return rc;
     const size_t qc = dev->nq_cap;
     std::vector<uint8_t> listed(dev->seg.size(), 0);
+    std::vector<uint64_t> seed_off(n, 0);
     segs.clear();
     StepDesc skip{};
     skip.op = OP_SKIP;
     skip.seg = -1;
+    uint64_t R = 0, seed = 0;
     for (uint32_t i = 0; i < n; i++) {
-      const Plan& p = plans[b[i]];
-      const uint32_t ns = uint32_t(p.steps.size());
-      dev->h_rs[i] = i;
+      const Entry& e = b[i];
+      const Plan& p = plans[e.q];
+      const uint32_t ns = uint32_t(p.steps.size()) - e.k0;
+      if (e.k0) {
+        const uint64_t elems = e.rows * p.steps[e.k0].in_cols;
+        if (seed + elems > dev->tcap) return WK_E_INVALID_ARG;  // the caller bounds the seeds
+        seed_off[i] = seed;
+        seed += elems;
+      }
+      dev->h_rs[i] = R;
+      R += e.k0 ? e.rows : 1;
       for (uint32_t k = 0; k < S; k++) {
         StepDesc* d = dev->h_desc + k * qc + i;
         if (k < ns) {
-          *d = p.steps[k];
+          *d = p.steps[e.k0 + k];
           d->final_step = (k + 1 == ns) ? 1 : 0;
-          d->in_rows = k == 0 ? 1 : 0;
-          d->in_off = 0;
+          d->in_rows = k ? 0 : e.k0 ? e.rows : 1;
+          d->in_off = k ? 0 : seed_off[i];
           d->out_rows = d->out_off = 0;
           if (d->seg >= 0 && !listed[size_t(d->seg)]) {
             listed[size_t(d->seg)] = 1;
@@ -259,16 +291,29 @@ struct HeavyLane {
         }
       }
     }
-    dev->h_rs[n] = n;
+    dev->h_rs[n] = R;
+    if (R > dev->rcap) return WK_E_INVALID_ARG;
     rc = ensure_segments(dev, segs);
     if (rc) return rc;
     for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
-    if (hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, ((S - 1) * qc + n) * sizeof(StepDesc),
-                                  hipMemcpyHostToDevice, dev->stream)) ||
-        hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                                  dev->stream)) ||
-        hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState), hipMemcpyHostToDevice,
-                                  dev->stream))) {
+    bool bad = hip_failed(hipMemcpyAsync(dev->d_desc, dev->h_desc, ((S - 1) * qc + n) * sizeof(StepDesc),
+                                         hipMemcpyHostToDevice, dev->stream)) ||
+               hip_failed(hipMemcpyAsync(dev->d_rs, dev->h_rs, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                         dev->stream)) ||
+               hip_failed(hipMemcpyAsync(dev->d_state, dev->h_state, (S + 1) * sizeof(StepState),
+                                         hipMemcpyHostToDevice, dev->stream));
+    if (seed && !bad) {
+      // the continued tables into the lane's first input buffer; the main
+      // lane's next work waits for the copies, as its iteration after next
+      // overwrites that buffer
+      for (uint32_t i = 0; i < n && !bad; i++)
+        if (b[i].k0)
+          bad = hip_failed(hipMemcpyAsync(dev->tbuf[1] + seed_off[i], main_tin + b[i].off,
+                                          b[i].rows * plans[b[i].q].steps[b[i].k0].in_cols * sizeof(uint32_t),
+                                          hipMemcpyDeviceToDevice, dev->stream));
+      bad = bad || hip_failed(hipEventRecord(seeded, dev->stream)) || hip_failed(hipStreamWaitEvent(main, seeded, 0));
+    }
+    if (bad) {
       unpin_segments(dev, segs);
       return WK_E_HIP;
     }
@@ -289,7 +334,8 @@ struct HeavyLane {
       unpin_segments(dev, segs);
       return WK_E_HIP;
     }
-    q = b;
+    q.clear();
+    for (const Entry& e : b) q.push_back(e.q);
     busy = true;
     return WK_OK;
   }
@@ -312,9 +358,20 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
   std::vector<QState> qs(nq);
   std::vector<uint64_t> fp(nq);
   for (uint32_t i = 0; i < nq; i++) fp[i] = plans[i].status ? 0 : project_footprint(*dev->st, plans[i]);
+  // projected bytes per row of a table entering step k (k >= 1, where the
+  // projection is linear in the rows: only a first step starts from a
+  // constant), for the check after each main-lane step
+  constexpr uint64_t kGainRows = uint64_t(1) << 20;
+  std::vector<std::vector<float>> gain(nq);
+  for (uint32_t i = 0; i < nq; i++) {
+    if (plans[i].status) continue;
+    gain[i].assign(plans[i].steps.size(), 0.f);
+    for (size_t k = 1; k < plans[i].steps.size(); k++)
+      gain[i][k] = float(double(project_footprint(*dev->st, plans[i], k, kGainRows)) / double(kGainRows));
+  }
   const uint32_t max_inflight = opts->per_query ? 1 : std::max<uint32_t>(1, opts->max_inflight);
   // queries projected above heavy_footprint run on the second lane
-  const uint64_t heavy_fp = opts->heavy_footprint ? opts->heavy_footprint : (uint64_t(16) << 20);
+  const uint64_t heavy_fp = opts->heavy_footprint ? opts->heavy_footprint : (uint64_t(1) << 20);
   const bool use_heavy = !opts->per_query && heavy_fp != ~uint64_t(0);
   const uint64_t budget = opts->footprint_budget ? opts->footprint_budget : dev->tcap * 4;
   if (opts->keep_results) {
@@ -415,6 +472,30 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     swap_lanes(dev);
     return r;
   };
+  // Launch a heavy batch: the continued queries `b`, then the oldest waiting
+  // heavy queries up to kHeavyBatch.  Queries whose segments do not fit the
+  // gstore beside the pinned ones go back: continued ones to the main lane
+  // (their tables are still there), waiting ones to run alone later.
+  auto launch_heavy = [&](std::vector<HeavyLane::Entry> b) -> int {
+    const size_t nc = b.size();
+    const size_t nw = std::min(heavy_queue.size(), kHeavyBatch - std::min(kHeavyBatch, nc));
+    for (size_t i = 0; i < nw; i++) b.push_back({heavy_queue[i], 0, 0, 0});
+    const int lr = heavy.launch(b, dev->tbuf[cur]);
+    if (lr && lr != WK_E_NOMEM_BUDGET && lr != WK_E_SEG_TOO_LARGE) return lr;
+    heavy_queue.erase(heavy_queue.begin(), heavy_queue.begin() + nw);
+    if (lr) {
+      for (size_t i = 0; i < nc; i++) {
+        inflight.push_back(b[i].q);
+        fp_used += fp[b[i].q];
+      }
+      for (size_t i = nc; i < b.size(); i++) retry.push_back(b[i].q);
+    } else if (dev->opts.profile) {
+      dev->kstats["serve:heavy_batches"].launches++;
+      dev->kstats["serve:heavy_queries"].launches += b.size();
+      dev->kstats["serve:heavy_continued"].launches += nc;
+    }
+    return WK_OK;
+  };
   int rc = WK_OK;
   while (ndone < nq && rc == WK_OK) {
     if (heavy.finished()) {
@@ -454,22 +535,8 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
       next++;
     }
     if (!heavy.busy && !heavy_queue.empty()) {
-      // the next heavy batch: the oldest waiting heavy queries
-      const size_t nb = std::min<size_t>(heavy_queue.size(), kHeavyBatch);
-      std::vector<uint32_t> b(heavy_queue.begin(), heavy_queue.begin() + nb);
-      const int lr = heavy.launch(b);
-      if (dev->opts.profile) {
-        dev->kstats["serve:heavy_batches"].launches++;
-        dev->kstats["serve:heavy_queries"].launches += nb;
-      }
-      if (lr == WK_OK || lr == WK_E_NOMEM_BUDGET || lr == WK_E_SEG_TOO_LARGE) {
-        heavy_queue.erase(heavy_queue.begin(), heavy_queue.begin() + nb);
-        // segments that do not fit beside the pinned ones: alone, later
-        if (lr) retry.insert(retry.end(), b.begin(), b.end());
-      } else {
-        rc = lr;
-        break;
-      }
+      rc = launch_heavy({});
+      if (rc) break;
     }
     if (inflight.empty()) {
       double wait = next < nq ? arrival_s[next] - now_s() : 1.0;
@@ -523,6 +590,29 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     }
     inflight.swap(still);
     cur ^= 1;
+    if (use_heavy && !heavy.busy && !inflight.empty()) {
+      // queries whose measured table projects past heavy_footprint continue
+      // on the second lane from that table (now in tbuf[cur])
+      std::vector<HeavyLane::Entry> b;
+      uint64_t seed = 0;
+      still.clear();
+      for (uint32_t q : inflight) {
+        const uint32_t k = qs[q].k;
+        const uint64_t elems = qs[q].rows * plans[q].steps[k].in_cols;
+        if (b.size() < kHeavyBatch && double(qs[q].rows) * gain[q][k] >= double(heavy_fp) &&
+            seed + elems + kHeavyBatch <= heavy.seed_cap()) {  // + the waiting queries' start rows
+          b.push_back({q, k, qs[q].rows, qs[q].off});
+          seed += elems;
+          fp_used -= fp[q];
+        } else {
+          still.push_back(q);
+        }
+      }
+      if (!b.empty()) {
+        inflight.swap(still);
+        rc = launch_heavy(std::move(b));
+      }
+    }
   }
   if (heavy.busy) {
     // stopped by an error: let the heavy batch finish before its segments
