@@ -66,7 +66,8 @@ enum {
   WK_E_HIP = 104,             /* a HIP runtime call failed; see wk_last_hip_error */
   WK_E_NO_DEVICE = 105,       /* no usable GPU                               */
   WK_E_OVERFLOW = 106,        /* a count exceeds the 64-bit/28-bit fields    */
-  WK_E_SEG_TOO_LARGE = 107,   /* one segment exceeds the gstore budget       */
+  WK_E_SEG_TOO_LARGE = 107,   /* a segment exceeds the gstore budget and no
+                                 pinned host copy of it could be made         */
   WK_E_STATE = 108,           /* call out of order (e.g. no result yet)      */
   WK_E_TOO_MANY_VARS = 109,   /* more variables than WK_MAX_VARS             */
   WK_E_WIRE = 110,            /* malformed archive, or a form not read here  */
@@ -187,6 +188,10 @@ typedef struct {
   uint64_t queries_run;
   uint64_t dense_segments;    /* segments with a direct key index */
   uint64_t dense_bytes;
+  uint64_t mapped_segments;   /* segments larger than the gstore budget, read in place
+                                 from pinned host memory */
+  uint64_t mapped_bytes;      /* their bytes (copied to pinned memory first when the store
+                                 is pageable) */
 } wk_dev_stats;
 
 /* Per-kernel timing (opts.profile = 1): name, launches, total ms on the

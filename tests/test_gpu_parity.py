@@ -244,6 +244,37 @@ def test_gstore_swapping(univ10, univ10_store, univ10_oracle, pinned):
     dev.close()
 
 
+@pytest.mark.parametrize("pinned,dense", [(True, 0), (True, -1), (False, -1)])
+def test_segments_larger_than_the_gstore_budget(univ10, univ10_store, univ10_oracle, pinned, dense):
+    """A 4 MiB gstore budget is below the two largest key sets (6.7 MB of
+    buckets each) and the index segment's values (4.2 MB): those segments are
+    read in place from pinned host memory (the store's own, or a pinned copy
+    of a pageable store's), the others swap through the budget, and every
+    result is exact per query, batched and served."""
+    store = univ10_store if pinned else wg.Store(univ10, pinned=False)
+    dev = wg.Device(store, gstore_budget=4 << 20, key_block_shift=10, value_block_shift=14,
+                    table_budget=256 << 20, dense_index=dense)
+    for f in (Q.chain3, Q.type_start2, Q.lubm_q1, Q.lubm_q7):
+        pats, nv = f()
+        check_query(dev, univ10_oracle, pats, nv)
+    stream = Q.template_stream(store, Q.univ_templates(), 200, seed=11)
+    qs = [(p, nv) for p, nv, _ in stream]
+    wants = [univ10_oracle.query(p, nv)[0] for p, nv in qs]
+    res = dev.batch(qs, use_graph=False)
+    for (pats, nv), (t, v2c, rc), want in zip(qs, res, wants):
+        assert rc == 0, (pats, rc)
+        assert same_rows(t, want), pats
+    arr = np.cumsum(np.random.default_rng(2).exponential(1 / 20000.0, size=len(qs)))
+    r = dev.serve(qs, arr, max_inflight=64)
+    assert (r["status"] == 0).all()
+    for i, want in enumerate(wants):
+        assert int(r["checksum"][i]) == (wg.checksum(want) if want.size else 0), i
+    s = dev.stats()
+    # at least the type index segment, which every index start reads
+    assert s["mapped_segments"] >= 1 and s["segments_evicted"] > 0
+    dev.close()
+
+
 @pytest.mark.parametrize("heavy", [0, 48 << 10])
 def test_serve_results_alone_and_in_mix(univ10_dev, univ10_oracle, univ10_store, heavy):
     """Continuous batching: each query's rows equal the oracle's (alone) and

@@ -45,6 +45,7 @@ void free_dev(wk_dev* d) {
     if (d->stage[k]) (void)hipHostFree(d->stage[k]);
     if (d->stage_ev[k]) (void)hipEventDestroy(d->stage_ev[k]);
   }
+  for (void* p : d->mapped_host) (void)hipHostFree(p);
   if (d->copy_done) (void)hipEventDestroy(d->copy_done);
   if (d->compute_done) (void)hipEventDestroy(d->compute_done);
   if (d->copy_stream) (void)hipStreamDestroy(d->copy_stream);
@@ -222,12 +223,9 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
     h.nb_magic = bucket_magic(m.num_buckets);
     h.total_buckets = uint32_t(m.total_buckets());
     h.edge_start = m.edge_start;
-    h.num_edges = uint32_t(m.num_edges);
     h.kb_base = uint32_t(kt);
     h.vb_base = uint32_t(vt);
-    h.resident = 0;
-    h.kphys = kNoPhys;
-    h.vphys = kNoPhys;
+    h.koff = h.voff = kNotResident;
     kt += s.nkb;
     vt += s.nvb;
   }

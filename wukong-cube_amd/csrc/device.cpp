@@ -146,9 +146,7 @@ int evict(wk_dev* dev, int v) {
   s.kblk.clear();
   s.vblk.clear();
   s.resident = false;
-  dev->h_segs[v].resident = 0;
-  dev->h_segs[v].kphys = kNoPhys;
-  dev->h_segs[v].vphys = kNoPhys;
+  dev->h_segs[v].koff = dev->h_segs[v].voff = kNotResident;
   dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
   WK_HIP(hipMemcpyAsync(dev->d_segs + v, dev->h_segs + v, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
@@ -187,19 +185,92 @@ int copy_in(wk_dev* dev, void* dst, const void* src, uint64_t n) {
   return WK_OK;
 }
 
+// Device address of byte `off` of a pinned host allocation at `base`.
+int mapped_addr(const void* base, uint64_t off, intptr_t* out) {
+  void* d = nullptr;
+  WK_HIP(hipHostGetDevicePointer(&d, const_cast<void*>(base), 0));
+  *out = reinterpret_cast<intptr_t>(d) + intptr_t(off);
+  return WK_OK;
+}
+
+// A segment larger than the whole gstore budget is read in place from pinned
+// host memory: the store's own when it is pinned, else a pinned copy made
+// once (kept until the device is destroyed).  Its buckets and edges then
+// cross the host link on every read; its direct key index, when it has one,
+// is built once into HBM, so dense-keyed segments read only edges there.
+int map_segment(wk_dev* dev, int si) {
+  SegSlot& s = dev->seg[size_t(si)];
+  const SegMeta& m = dev->st->segs[size_t(si)];
+  const uint64_t kbytes = m.total_buckets() * ASSOCIATIVITY * sizeof(Slot);
+  const uint64_t vbytes = m.num_edges * sizeof(uint32_t);
+  const void* kbase = dev->st->slots;
+  const void* vbase = dev->st->values;
+  uint64_t koff = m.bucket_start * ASSOCIATIVITY * sizeof(Slot), voff = m.edge_start * sizeof(uint32_t);
+  if (!dev->st->pinned) {
+    void* kp = nullptr;
+    void* vp = nullptr;
+    if (hipHostMalloc(&kp, std::max<uint64_t>(kbytes, 1), hipHostMallocDefault) != hipSuccess)
+      return WK_E_SEG_TOO_LARGE;
+    dev->mapped_host.push_back(kp);
+    if (hipHostMalloc(&vp, std::max<uint64_t>(vbytes, 1), hipHostMallocDefault) != hipSuccess)
+      return WK_E_SEG_TOO_LARGE;
+    dev->mapped_host.push_back(vp);
+    const uint8_t* ks = reinterpret_cast<const uint8_t*>(dev->st->slots) + koff;
+    const uint8_t* vs = reinterpret_cast<const uint8_t*>(dev->st->values) + voff;
+    constexpr uint64_t kPiece = uint64_t(1) << 20;
+#pragma omp parallel for schedule(static)
+    for (uint64_t o = 0; o < kbytes; o += kPiece)
+      std::memcpy(static_cast<uint8_t*>(kp) + o, ks + o, std::min(kPiece, kbytes - o));
+#pragma omp parallel for schedule(static)
+    for (uint64_t o = 0; o < vbytes; o += kPiece)
+      std::memcpy(static_cast<uint8_t*>(vp) + o, vs + o, std::min(kPiece, vbytes - o));
+    kbase = kp;
+    vbase = vp;
+    koff = voff = 0;
+  }
+  intptr_t kd = 0, vd = 0;
+  int rc = mapped_addr(kbase, koff, &kd);
+  if (!rc) rc = mapped_addr(vbase, voff, &vd);
+  if (rc) return rc;
+  // offsets from the arenas, in buckets and edges (both sides are aligned to them)
+  const intptr_t dk = kd - reinterpret_cast<intptr_t>(dev->karena);
+  const intptr_t dv = vd - reinterpret_cast<intptr_t>(dev->varena);
+  constexpr intptr_t kBucketBytes = ASSOCIATIVITY * sizeof(Slot);
+  if (dk % kBucketBytes != 0 || dv % intptr_t(sizeof(uint32_t)) != 0) return WK_E_SEG_TOO_LARGE;
+  DevSeg& hs = dev->h_segs[si];
+  hs.koff = dk / kBucketBytes;
+  hs.voff = dv / intptr_t(sizeof(uint32_t));
+  dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
+  if (dev->dense[size_t(si)] && !dev->dense_built[size_t(si)]) {
+    hs.dvmin = m.vmin;
+    hs.dvn = m.vmax - m.vmin + 1;
+    WK_HIP(hipMemsetAsync(dev->dense[size_t(si)], 0, uint64_t(hs.dvn) * sizeof(uint64_t), dev->copy_stream));
+    launch_dense_build(dev->karena, dev->d_ktab, dev->kshift, hs, dev->dense[size_t(si)], dev->copy_stream);
+    hs.dense = dev->dense[size_t(si)];
+    dev->dense_built[size_t(si)] = 1;
+  }
+  WK_HIP(hipMemcpyAsync(dev->d_segs + si, dev->h_segs + si, sizeof(DevSeg), hipMemcpyHostToDevice,
+                        dev->copy_stream));
+  s.resident = s.mapped = true;
+  s.last_use = ++dev->tick;
+  dev->stats.mapped_segments++;
+  dev->stats.mapped_bytes += kbytes + vbytes;
+  return WK_OK;
+}
+
 int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
   SegSlot& s = dev->seg[size_t(si)];
   if (s.resident) {
     s.last_use = ++dev->tick;
     return WK_OK;
   }
-  if (s.nkb > dev->nkblk || s.nvb > dev->nvblk) return WK_E_SEG_TOO_LARGE;
+  if (s.nkb > dev->nkblk || s.nvb > dev->nvblk) return map_segment(dev, si);
   while (dev->free_k.size() < s.nkb || dev->free_v.size() < s.nvb) {
     int victim = -1;
     uint64_t best = ~uint64_t(0);
     for (size_t i = 0; i < dev->seg.size(); i++) {
       const SegSlot& c = dev->seg[i];
-      if (!c.resident || c.pins > 0 || int(i) == si) continue;
+      if (!c.resident || c.mapped || c.pins > 0 || int(i) == si) continue;
       if (std::find(keep.begin(), keep.end(), int(i)) != keep.end()) continue;
       if (c.last_use < best) {
         best = c.last_use;
@@ -247,13 +318,14 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
   if (s.nvb)
     WK_HIP(hipMemcpyAsync(dev->d_vtab + hs.vb_base, dev->h_vtab + hs.vb_base, s.nvb * sizeof(uint32_t),
                           hipMemcpyHostToDevice, dev->copy_stream));
-  auto first_if_consecutive = [](const std::vector<uint32_t>& blk) {
+  // consecutive blocks: addressed from the first one's offset
+  auto offset_if_consecutive = [](const std::vector<uint32_t>& blk, uint32_t shift) {
     for (size_t j = 1; j < blk.size(); j++)
-      if (blk[j] != blk[0] + j) return kNoPhys;
-    return blk.empty() ? 0u : blk[0];
+      if (blk[j] != blk[0] + j) return kBlockTable;
+    return blk.empty() ? int64_t(0) : int64_t(blk[0]) << shift;
   };
-  hs.kphys = first_if_consecutive(s.kblk);
-  hs.vphys = first_if_consecutive(s.vblk);
+  hs.koff = offset_if_consecutive(s.kblk, dev->kshift);
+  hs.voff = offset_if_consecutive(s.vblk, dev->vshift);
   if (dev->dense[si] && !dev->dense_built[si]) {
     // the direct key index from the buckets just copied (copy stream, so the
     // compute stream sees it once it waits for the copies)
@@ -264,7 +336,6 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
     hs.dense = dev->dense[si];
     dev->dense_built[si] = 1;
   }
-  hs.resident = 1;
   WK_HIP(hipMemcpyAsync(dev->d_segs + si, dev->h_segs + si, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
   s.resident = true;

@@ -52,6 +52,7 @@ struct SegSlot {
   uint32_t nkb = 0, nvb = 0;      // blocks needed
   std::vector<uint32_t> kblk, vblk;
   bool resident = false;
+  bool mapped = false;            // read in place from pinned host memory (never evicted)
   uint64_t last_use = 0;
   int pins = 0;
 };
@@ -167,6 +168,9 @@ struct wk_dev : wk::LaneState {
   uint8_t stage_used[wk::kStageBufs] = {};
   int stage_next = 0;
   uint64_t stage_bytes = 0;
+  // pinned host copies of segments larger than the gstore budget, made when
+  // the store itself is pageable; they are read in place by the kernels
+  std::vector<void*> mapped_host;
 
   // compiled plans of the last batch (storage reused by the next)
   std::vector<wk::Plan> plan_cache;

@@ -18,15 +18,16 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kPersistentGrid = 2048;
 
-// A segment whose blocks sit consecutively in the arena (every segment of a
-// store that fits the budget) is addressed directly; otherwise through its
-// block table, one more dependent load.
-// Both choose an arena index, not a pointer: the two ways meet on an integer,
-// so no pointer phi reaches the load.
+// A segment whose buckets (edges) are contiguous, in consecutive arena blocks
+// or in pinned host memory, is addressed directly from its offset; otherwise
+// through its block table, one more dependent load.  Both choose an index
+// relative to the arena, not a pointer: the two ways meet on an integer, so
+// no pointer phi reaches the load.  A host-resident segment's offset is
+// negative or beyond the arena; the index arithmetic is modulo 2^64.
 __device__ __forceinline__ const Slot* bucket_addr(const StepArgs& a, const DevSeg& s, uint64_t b) {
   uint64_t i;
-  if (s.kphys != kNoPhys) {
-    i = (uint64_t(s.kphys) << a.kshift) + b;
+  if (s.koff > kBlockTable) {
+    i = uint64_t(s.koff) + b;
   } else {
     const uint64_t blk = a.ktab[s.kb_base + (b >> a.kshift)];
     i = (blk << a.kshift) | (b & ((uint64_t(1) << a.kshift) - 1));
@@ -36,8 +37,8 @@ __device__ __forceinline__ const Slot* bucket_addr(const StepArgs& a, const DevS
 
 __device__ __forceinline__ uint32_t edge_at(const StepArgs& a, const DevSeg& s, uint64_t e) {
   uint64_t i;
-  if (s.vphys != kNoPhys) {
-    i = (uint64_t(s.vphys) << a.vshift) + e;
+  if (s.voff > kBlockTable) {
+    i = uint64_t(s.voff) + e;
   } else {
     const uint64_t blk = a.vtab[s.vb_base + (e >> a.vshift)];
     i = (blk << a.vshift) | (e & ((uint64_t(1) << a.vshift) - 1));
@@ -302,8 +303,8 @@ __device__ __forceinline__ bool search_sorted(At at, uint64_t n, uint32_t x) {
 // boolean answer leaves the branch.
 __device__ __forceinline__ bool contains(const StepArgs& a, const DevSeg& s, uint64_t off, uint64_t n,
                                          uint32_t x) {
-  if (s.vphys != kNoPhys) {
-    const uint32_t* e = a.varena + (uint64_t(s.vphys) << a.vshift) + off;
+  if (s.voff > kBlockTable) {
+    const uint32_t* e = a.varena + (uint64_t(s.voff) + off);
     return search_sorted([e](uint64_t i) { return e[i]; }, n, x);
   }
   const uint64_t mask = (uint64_t(1) << a.vshift) - 1;
@@ -370,7 +371,7 @@ __device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d,
   // KEY_TYPE_INDEX reads [0|t|IN]: only type/predicate ids fit the 17-bit pid field
   const bool tix = d.key_mode == KEY_TYPE_INDEX;
   const uint64_t key = tix ? make_key(0, kv, DIR_IN) : make_key(kv, d.pid, d.dir);
-  const bool valid = s.resident != 0 && (!tix || kv < (1u << NBITS_IDX));
+  const bool valid = seg_resident(s) && (!tix || kv < (1u << NBITS_IDX));
   uint64_t ptr = 0;
   bool hit;
   if (s.dense && !tix) {
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
       const uint64_t r0 = a.row_start[q];
       if constexpr (kPasses % 2 == 0) {
         // rows g and g + G of each pair of passes together
-        const bool res = live && s.resident != 0;
+        const bool res = live && seg_resident(s);
         for (int pass = 0; pass < kPasses; pass += 2) {
           const uint64_t g = tlo + uint64_t(pass) * G + grp;
           if (g < thi) {
@@ -1517,8 +1518,8 @@ __global__ __launch_bounds__(kBlock) void k_dense_build(const Slot* karena, cons
   const uint64_t mask = (uint64_t(1) << kshift) - 1;
   for (uint64_t b = uint64_t(blockIdx.x) * kBlock + threadIdx.x; b < s.total_buckets;
        b += uint64_t(gridDim.x) * kBlock) {
-    const uint64_t i = s.kphys != kNoPhys ? (uint64_t(s.kphys) << kshift) + b
-                                          : (uint64_t(ktab[s.kb_base + (b >> kshift)]) << kshift) | (b & mask);
+    const uint64_t i = s.koff > kBlockTable ? uint64_t(s.koff) + b
+                                            : (uint64_t(ktab[s.kb_base + (b >> kshift)]) << kshift) | (b & mask);
     const Slot* bk = karena + i * ASSOCIATIVITY;
     for (int k = 0; k < ASSOCIATIVITY - 1; ++k) {
       const Slot sl = bk[k];

@@ -30,20 +30,22 @@ struct DevSeg {
   uint64_t edge_start;    // global edge offset of the segment (ptr.off - edge_start = local)
   const uint64_t* dense;  // direct key index of a segment whose key vids are dense: the ptr
                           // word of key vid dvmin + i at dense[i], 0 for no key; or null
+  int64_t koff;           // where bucket 0 is, in buckets from the key arena, when the
+                          // segment's buckets are contiguous (consecutive key blocks, or a
+                          // segment read in place from pinned host memory); kBlockTable when
+                          // they are found through the key-block table; kNotResident
+  int64_t voff;           // the same for edge 0, in edges from the value arena
   uint32_t num_buckets;   // main buckets: hash modulus
   uint32_t total_buckets; // main + extended: bound on chain links
-  uint32_t num_edges;
   uint32_t kb_base;       // first entry in the key-block table
   uint32_t vb_base;       // first entry in the value-block table
-  uint32_t resident;
-  uint32_t kphys;         // first physical key block when the segment's key blocks are
-                          // consecutive in the arena, else kNoPhys
-  uint32_t vphys;         // same for the value blocks
   uint32_t dvmin;         // first vid the direct index covers
   uint32_t dvn;           // vids it covers (>= 1 when dense is set)
 };
 static_assert(sizeof(DevSeg) == 64, "DevSeg is one 64-byte line");
-constexpr uint32_t kNoPhys = 0xFFFFFFFFu;
+constexpr int64_t kNotResident = INT64_MIN;
+constexpr int64_t kBlockTable = INT64_MIN + 1;
+__host__ __device__ inline bool seg_resident(const DevSeg& s) { return s.koff != kNotResident; }
 // StepDesc.out_off (and the arena offset in qres) of a final table that did
 // not fit the result arena: the query reports WK_E_NOMEM_BUDGET, the others
 // of the step go on.

@@ -79,7 +79,7 @@ class DevStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "h2d_bytes", "segments_loaded", "segments_evicted", "key_blocks_total", "key_blocks_free",
         "value_blocks_total", "value_blocks_free", "table_capacity_elems", "queries_run", "dense_segments",
-        "dense_bytes")]
+        "dense_bytes", "mapped_segments", "mapped_bytes")]
 
 
 class KernelStat(C.Structure):
