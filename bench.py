@@ -135,18 +135,24 @@ def make_workload(args, wg, Q):
 
 
 def run_single(dev, singles, args):
+    """Each single query `steps` times with profiling off (latency), then
+    `steps` times with every kernel timed (per-kernel times and bytes)."""
     single = {}
     for name, (pats, nv) in singles.items():
+        dev.set_profile(False)
         for _ in range(args.warmup):
             dev.query(pats, nv, fetch=False)
         dev.sync()
-        dev.reset_kernel_stats()
         lat = []
         for _ in range(args.steps):
             t = time.perf_counter()
             _, _, rc = dev.query(pats, nv, fetch=False)
             lat.append((time.perf_counter() - t) * 1e6)
             assert rc == 0
+        dev.set_profile(True)
+        dev.reset_kernel_stats()
+        for _ in range(args.steps):
+            dev.query(pats, nv, fetch=False)
         ks = dev.kernel_stats()
         table, v2c, rc = dev.query(pats, nv, fetch=True)
         single[name] = {"patterns": len(pats), "rows": int(table.shape[0]), "cols": int(table.shape[1]),
@@ -367,10 +373,19 @@ def main():
                     probe_width=args.probe_width, rows_per_wave=args.rows_per_wave, dense_index=args.dense_index)
     t0 = time.time()
     dev.load_all()
+    load_s = time.time() - t0
     st = dev.stats()
-    log("gstore loaded in %.1fs (%.2f GB H2D, %d key / %d value blocks free; direct key indexes for %d "
-        "segments, %.2f GB)" % (time.time() - t0, st["h2d_bytes"] / 1e9, st["key_blocks_free"],
-                                st["value_blocks_free"], st["dense_segments"], st["dense_bytes"] / 1e9))
+    # swaps come by DMA from a pinned store, else through the pinned staging ring
+    gstore = {"host_bytes": info["num_buckets"] * 128 + info["num_values"] * 4, "pinned": int(info["pinned"]),
+              "staging_ring": int(not info["pinned"]), "load_all_s": load_s,
+              "load_all_h2d_bytes": st["h2d_bytes"], "load_all_h2d_gbps": st["h2d_bytes"] / load_s / 1e9,
+              "mapped_segments": st["mapped_segments"]}
+    log("gstore loaded in %.1fs (%.2f GB H2D, %.1f GB/s, %s; %d key / %d value blocks free; direct key indexes "
+        "for %d segments, %.2f GB; %d segments read in place from host memory)" % (
+            load_s, st["h2d_bytes"] / 1e9, gstore["load_all_h2d_gbps"],
+            "pinned store" if info["pinned"] else "pageable store through the staging ring",
+            st["key_blocks_free"], st["value_blocks_free"], st["dense_segments"], st["dense_bytes"] / 1e9,
+            st["mapped_segments"]))
 
     out = {"metric": "queries/sec + p50/p99 latency (batched vs per-query GPU); edges/s; HBM GB/s"}
 
@@ -622,7 +637,7 @@ def main():
         "edges_per_s": (edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values())) if single else None,
         "serve": serve, "partition": partition, "final_process": final, "roofline": roofline,
         "cpu_baseline": cpu, "parity_gate": gate,
-        "gstore": {"h2d_bytes": dev.stats()["h2d_bytes"], "segments_evicted": dev.stats()["segments_evicted"]},
+        "gstore": dict(gstore, h2d_bytes=dev.stats()["h2d_bytes"], segments_evicted=dev.stats()["segments_evicted"]),
     })
     dev.close()
     print(json.dumps(out))

@@ -467,30 +467,48 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
 
 // ----------------------------------------------------------------- timing
 
+namespace {
+
+hipEvent_t take_event(wk_dev* dev) {
+  if (!dev->event_pool.empty()) {
+    hipEvent_t e = dev->event_pool.back();
+    dev->event_pool.pop_back();
+    return e;
+  }
+  // timing only: no system-scope fence when the event completes, which
+  // would add an L2 write-back to the measured kernel
+  hipEvent_t e = nullptr;
+  return hipEventCreateWithFlags(&e, hipEventDisableSystemFence) == hipSuccess ? e : nullptr;
+}
+
+}  // namespace
+
+// The launches between timed_begin and timed_end take their start and end
+// times from their own dispatch packets (arm_launch_timer).
 void timed_begin(wk_dev* dev, hipEvent_t* ev) {
   *ev = nullptr;
   if (!dev->opts.profile) return;
-  if (dev->event_pool.empty()) {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    dev->event_pool.push_back(e);
+  hipEvent_t a = take_event(dev), b = a ? take_event(dev) : nullptr;
+  if (!b) {
+    if (a) dev->event_pool.push_back(a);
+    return;
   }
-  *ev = dev->event_pool.back();
-  dev->event_pool.pop_back();
-  (void)hipEventRecord(*ev, dev->stream);
+  *ev = a;
+  dev->timing_stop = b;
+  arm_launch_timer(a, b);
 }
 
 void timed_end(wk_dev* dev, hipEvent_t a, const char* name, double bytes) {
   if (!a) return;
-  hipEvent_t b;
-  if (dev->event_pool.empty()) {
-    if (hipEventCreate(&b) != hipSuccess) return;
+  hipEvent_t b = dev->timing_stop;
+  dev->timing_stop = nullptr;
+  if (disarm_launch_timer()) {
+    dev->pending.push_back(PendingTiming{name, a, b, bytes});
   } else {
-    b = dev->event_pool.back();
-    dev->event_pool.pop_back();
+    // nothing was launched
+    dev->event_pool.push_back(a);
+    dev->event_pool.push_back(b);
   }
-  (void)hipEventRecord(b, dev->stream);
-  dev->pending.push_back(PendingTiming{name, a, b, bytes});
 }
 
 double host_now() {

@@ -197,6 +197,7 @@ struct wk_dev : wk::LaneState {
   std::map<std::string, wk::KernelStat> kstats;
   std::vector<wk::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
+  hipEvent_t timing_stop = nullptr;  // the open timed_begin's end event
 };
 
 namespace wk {

@@ -10,9 +10,47 @@
 // (:328-447) use binary search: edge lists are sorted and unique
 // (loader/base_loader.hpp:481-498), so the result equals the linear scan's.
 // All kernels handle many queries at once through per-query descriptors.
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 
 namespace wk {
+
+namespace {
+
+// Armed by the library's profiling mode around a group of launches (see
+// arm_launch_timer).  Launches then take their times from their own dispatch
+// packets: host launch latency, which a pair of stream events around a launch
+// to an idle GPU would count, is left out.
+struct LaunchTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool used = false;
+};
+thread_local LaunchTimer t_timer;
+
+}  // namespace
+
+void arm_launch_timer(hipEvent_t start, hipEvent_t stop) { t_timer = LaunchTimer{start, stop, false}; }
+
+bool disarm_launch_timer() {
+  const bool used = t_timer.used;
+  t_timer = LaunchTimer{};
+  return used;
+}
+
+// Every kernel launch of the library: the first launch of an armed group
+// records its start into `start`, each records its end into `stop` (the last
+// one's stays).
+#define WK_LAUNCH(K, G, B, S, ST, ...)                                                        \
+  do {                                                                                        \
+    if (t_timer.stop) {                                                                       \
+      hipExtLaunchKernelGGL(K, G, B, S, ST, t_timer.used ? nullptr : t_timer.start, t_timer.stop, \
+                            0u, __VA_ARGS__);                                                 \
+      t_timer.used = true;                                                                    \
+    } else {                                                                                  \
+      hipLaunchKernelGGL(K, G, B, S, ST, __VA_ARGS__);                                        \
+    }                                                                                         \
+  } while (0)
 namespace {
 
 constexpr int kWave = 64;
@@ -1563,9 +1601,9 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
 template <int W>
 void probe_launch(const StepArgs& a, unsigned grid, bool one_query, hipStream_t s) {
   if (one_query)
-    hipLaunchKernelGGL((k_probe<W, 256>), dim3(grid), dim3(256), 0, s, a);
+    WK_LAUNCH((k_probe<W, 256>), dim3(grid), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((k_probe<W, 1024>), dim3(grid), dim3(1024), 0, s, a);
+    WK_LAUNCH((k_probe<W, 1024>), dim3(grid), dim3(1024), 0, s, a);
 }
 
 void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
@@ -1581,22 +1619,22 @@ void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
 }
 
 void launch_scan_tiles(const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, a);
+  WK_LAUNCH(k_scan_tiles, dim3(1), dim3(1024), 0, s, a);
 }
 
 void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
   const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
-  hipLaunchKernelGGL(k_scan_rows, dim3(grid), dim3(kBlock), 0, s, a);
+  WK_LAUNCH(k_scan_rows, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 void launch_scan_queries(const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
+  WK_LAUNCH(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
 }
 
 void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
   const uint32_t nqb = (a.nq + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_query_sums, dim3(nqb + kPersistentGrid / 4), dim3(kBlock), 0, s, a, nqb);
-  hipLaunchKernelGGL(k_scan_queries_mb, dim3(nqb), dim3(kBlock), 0, s, a, nqb);
+  WK_LAUNCH(k_query_sums, dim3(nqb + kPersistentGrid / 4), dim3(kBlock), 0, s, a, nqb);
+  WK_LAUNCH(k_scan_queries_mb, dim3(nqb), dim3(kBlock), 0, s, a, nqb);
 }
 
 void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s) {
@@ -1620,7 +1658,7 @@ void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
 void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s) {
   const uint64_t tiles = (out_hint + kOutTile - 1) / kOutTile;
   const unsigned grid = out_hint ? grid_for((tiles + kBlock - 1) / kBlock, 1u << 20) : kPersistentGrid;
-  hipLaunchKernelGGL(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, a);
+  WK_LAUNCH(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
@@ -1629,18 +1667,18 @@ void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint,
         out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : 4 * kPersistentGrid;
     // the checksum variant only when final tables are checksummed (serving)
     if (a.ck)
-      hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(kBlock), 0, s, a);
+      WK_LAUNCH(k_gather<true>, dim3(grid), dim3(kBlock), 0, s, a);
     else
-      hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(kBlock), 0, s, a);
+      WK_LAUNCH(k_gather<false>, dim3(grid), dim3(kBlock), 0, s, a);
     return;
   }
   // rows per wavefront -> lanes per row
   const unsigned grid = kPersistentGrid * 2;
   switch (rows_per_wave) {
-    case 64: hipLaunchKernelGGL(k_gather_rows<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(k_gather_rows<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_gather_rows<64>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 64: WK_LAUNCH(k_gather_rows<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 16: WK_LAUNCH(k_gather_rows<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: WK_LAUNCH(k_gather_rows<16>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: WK_LAUNCH(k_gather_rows<64>, dim3(grid), dim3(kBlock), 0, s, a); break;
   }
 }
 
@@ -1652,7 +1690,7 @@ void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipS
 void launch_checksum(const uint32_t* base, const CkDesc* d, uint32_t n, uint64_t total_rows, uint64_t* sums,
                      hipStream_t s) {
   if (n == 0 || total_rows == 0) return;
-  hipLaunchKernelGGL(k_checksum, dim3(grid_for((total_rows + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
+  WK_LAUNCH(k_checksum, dim3(grid_for((total_rows + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
                      base, d, n, total_rows, reinterpret_cast<unsigned long long*>(sums));
 }
 
@@ -1671,20 +1709,20 @@ uint64_t* launch_partition(const uint32_t* in, uint64_t rows, uint32_t cols, uin
   uint64_t* csum = hist + n;
   uint64_t* heads = csum + nch;
   const unsigned grid = grid_for(ntiles, 8192);
-  hipLaunchKernelGGL(k_part_hist, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, hist);
-  hipLaunchKernelGGL(k_part_scan_local, dim3(grid_for(nch, 8192)), dim3(kBlock), 0, s, hist, n, csum);
-  hipLaunchKernelGGL(k_part_scan_chunks, dim3(1), dim3(1024), 0, s, (const uint64_t*)hist, csum, nch, ntiles,
+  WK_LAUNCH(k_part_hist, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, hist);
+  WK_LAUNCH(k_part_scan_local, dim3(grid_for(nch, 8192)), dim3(kBlock), 0, s, hist, n, csum);
+  WK_LAUNCH(k_part_scan_chunks, dim3(1), dim3(1024), 0, s, (const uint64_t*)hist, csum, nch, ntiles,
                      nparts, heads);
   const uint64_t* h = hist;
   const uint64_t* c = csum;
   if (cols <= 2)
-    hipLaunchKernelGGL(k_part_scatter<2>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
+    WK_LAUNCH(k_part_scatter<2>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   else if (cols <= 4)
-    hipLaunchKernelGGL(k_part_scatter<4>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
+    WK_LAUNCH(k_part_scatter<4>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   else if (cols <= 8)
-    hipLaunchKernelGGL(k_part_scatter<8>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
+    WK_LAUNCH(k_part_scatter<8>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   else
-    hipLaunchKernelGGL(k_part_scatter<0>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
+    WK_LAUNCH(k_part_scatter<0>, dim3(grid), dim3(kBlock), 0, s, in, rows, cols, col, f, h, c, out);
   return heads;
 }
 
@@ -1747,38 +1785,38 @@ __global__ __launch_bounds__(kBlock) void k_fp_project(const uint32_t* table, ui
 void launch_fp_keys(const uint32_t* table, uint32_t cols, uint32_t c, uint32_t mode, const uint32_t* prev, uint64_t n,
                     uint32_t* pairs, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_fp_keys, dim3(grid_for((n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s, table, cols, c,
+  WK_LAUNCH(k_fp_keys, dim3(grid_for((n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s, table, cols, c,
                      mode, prev, n, pairs);
 }
 
 void launch_fp_dedup(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted, uint64_t n,
                      uint32_t* pairs, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_fp_dedup, dim3(grid_for((n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s, table, cols,
+  WK_LAUNCH(k_fp_dedup, dim3(grid_for((n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s, table, cols,
                      req, sorted, n, pairs);
 }
 
 void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req, const uint32_t* sorted,
                        uint64_t first, uint64_t n, uint32_t* out, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_fp_project, dim3(grid_for((n * req.n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
+  WK_LAUNCH(k_fp_project, dim3(grid_for((n * req.n + kBlock - 1) / kBlock, 8192)), dim3(kBlock), 0, s,
                      table, cols, req, sorted, first, n, out);
 }
 
 void launch_dense_build(const Slot* karena, const uint32_t* ktab, uint32_t kshift, const DevSeg& s, uint64_t* dense,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_dense_build, dim3(grid_for((uint64_t(s.total_buckets) + kBlock - 1) / kBlock, 8192)),
+  WK_LAUNCH(k_dense_build, dim3(grid_for((uint64_t(s.total_buckets) + kBlock - 1) / kBlock, 8192)),
                      dim3(kBlock), 0, st, karena, ktab, kshift, s, dense);
 }
 
 void launch_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C, uint64_t T, uint32_t n,
                         uint64_t* out, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_window_rows, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pfx, R, j0, C, T, n, out);
+  WK_LAUNCH(k_window_rows, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pfx, R, j0, C, T, n, out);
 }
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((n + 255) / 256, 4096)), dim3(256), 0, s, p, n, v);
+  WK_LAUNCH(k_fill_u64, dim3(grid_for((n + 255) / 256, 4096)), dim3(256), 0, s, p, n, v);
 }
 
 }  // namespace wk

@@ -189,6 +189,14 @@ void launch_fp_project(const uint32_t* table, uint32_t cols, const ColList& req,
                        uint64_t first, uint64_t n, uint32_t* out, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 
+// Kernel timing for profiling: between arm_launch_timer and
+// disarm_launch_timer, launches made on this thread take their times from
+// their own dispatch packets (hipExtLaunchKernelGGL): `start` from the first
+// launch, `stop` from each (the last one's stays).  disarm_launch_timer
+// returns whether any launch took them.
+void arm_launch_timer(hipEvent_t start, hipEvent_t stop);
+bool disarm_launch_timer();
+
 // Direct key index of a resident segment s (its buckets at karena through
 // ktab, s.dense/dvmin/dvn set): dense[vid - dvmin] = ptr of every key in the
 // segment's buckets.  dense must be zeroed before.
