@@ -307,7 +307,8 @@ typedef struct {
   uint32_t* cols;     /* [nq] result columns */
   int32_t* status;    /* [nq] per-query status code */
   uint64_t* offset;   /* [nq] element offset of the table in the batch arena */
-  int32_t* v2c;       /* [nq * WK_MAX_VARS] var->col maps (-1 = unbound) */
+  int32_t* v2c;       /* [nq * WK_MAX_VARS] var->col maps (-1 = unbound): row i holds query
+                         i's map in its first nvars entries; the rest are not written */
 } wk_batch_result;
 
 /* Run nq queries together: each pattern step of all queries is one set of

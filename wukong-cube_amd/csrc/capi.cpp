@@ -194,6 +194,7 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   if (o->device < 0 || o->device >= n) return WK_E_INVALID_ARG;
   d->device = o->device;
   WK_HIP(hipSetDevice(d->device));
+  init_launch_grids();
   // the no-allocation entry points' fixed scratch on this device
   int src = noalloc_init(d->device);
   if (src) return src;
@@ -777,10 +778,13 @@ int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph
   for (uint32_t i = 0; i < nq; i++) {
     compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
     out->cols[i] = plans[i].cols;
-    if (out->v2c)
-      for (int32_t v = 0; v < WK_MAX_VARS; v++)
-        out->v2c[size_t(i) * WK_MAX_VARS + v] =
-            (plans[i].status == WK_OK && v < qs[i].nvars) ? plans[i].v2c[size_t(v)] : -1;
+    if (out->v2c) {
+      // the query's own variables only (a full row of WK_MAX_VARS entries
+      // per query was a megabyte of host writes per 4,096-query batch)
+      const int32_t nv = std::min<int32_t>(std::max<int32_t>(qs[i].nvars, 0), WK_MAX_VARS);
+      int32_t* m = out->v2c + size_t(i) * WK_MAX_VARS;
+      for (int32_t v = 0; v < nv; v++) m[v] = plans[i].status == WK_OK ? plans[i].v2c[size_t(v)] : -1;
+    }
   }
   host_phase(dev, "host:batch_compile", t0);
   uint64_t end = 0;

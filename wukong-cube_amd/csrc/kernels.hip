@@ -12,6 +12,8 @@
 // All kernels handle many queries at once through per-query descriptors.
 #include <hip/hip_ext.h>
 
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace wk {
@@ -1596,7 +1598,64 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
   return unsigned(units < cap ? units : cap);
 }
 
+// Grids of the launches whose row counts live on the device (batched steps
+// and serving iterations): the workgroups resident on the whole device at
+// once, whose workgroups then stride over the tiles.  A larger grid only adds
+// rounds of workgroups that find no tile: the batch probe's 1,024-thread
+// workgroup holds 120 KB of LDS, one per CU, so 2,048 of them took 8 rounds.
+struct LaunchGrids {
+  unsigned probe[4];       // k_probe<W, 1024>, W = 1, 2, 4, 8
+  unsigned scan_rows;
+  unsigned tile_rows;
+  unsigned gather[2];      // k_gather<false>, k_gather<true>
+  unsigned gather_rows[4]; // k_gather_rows<1, 4, 16, 64>
+  unsigned query_sums;
+};
+LaunchGrids g_grids = {{kPersistentGrid, kPersistentGrid, kPersistentGrid, kPersistentGrid},
+                       kPersistentGrid,
+                       kPersistentGrid,
+                       {4 * kPersistentGrid, 4 * kPersistentGrid},
+                       {2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid},
+                       kPersistentGrid};
+std::once_flag g_grids_once;
+
+unsigned resident_grid(const void* kernel, int block, int cus, unsigned fallback) {
+  int per = 0;
+  if (cus <= 0 || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) != hipSuccess || per <= 0) {
+    (void)hipGetLastError();
+    return fallback;
+  }
+  return unsigned(cus) * unsigned(per);
+}
+
 }  // namespace
+
+void init_launch_grids() {
+  std::call_once(g_grids_once, [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    auto rg = [cus](const void* k, int block, unsigned fallback) { return resident_grid(k, block, cus, fallback); };
+    LaunchGrids g = g_grids;
+    g.probe[0] = rg(reinterpret_cast<const void*>(k_probe<1, 1024>), 1024, g.probe[0]);
+    g.probe[1] = rg(reinterpret_cast<const void*>(k_probe<2, 1024>), 1024, g.probe[1]);
+    g.probe[2] = rg(reinterpret_cast<const void*>(k_probe<4, 1024>), 1024, g.probe[2]);
+    g.probe[3] = rg(reinterpret_cast<const void*>(k_probe<8, 1024>), 1024, g.probe[3]);
+    g.scan_rows = rg(reinterpret_cast<const void*>(k_scan_rows), kBlock, g.scan_rows);
+    g.tile_rows = rg(reinterpret_cast<const void*>(k_tile_rows), kBlock, g.tile_rows);
+    g.gather[0] = rg(reinterpret_cast<const void*>(k_gather<false>), kBlock, g.gather[0]);
+    g.gather[1] = rg(reinterpret_cast<const void*>(k_gather<true>), kBlock, g.gather[1]);
+    g.gather_rows[0] = rg(reinterpret_cast<const void*>(k_gather_rows<1>), kBlock, g.gather_rows[0]);
+    g.gather_rows[1] = rg(reinterpret_cast<const void*>(k_gather_rows<4>), kBlock, g.gather_rows[1]);
+    g.gather_rows[2] = rg(reinterpret_cast<const void*>(k_gather_rows<16>), kBlock, g.gather_rows[2]);
+    g.gather_rows[3] = rg(reinterpret_cast<const void*>(k_gather_rows<64>), kBlock, g.gather_rows[3]);
+    g.query_sums = rg(reinterpret_cast<const void*>(k_query_sums), kBlock, g.query_sums);
+    g_grids = g;
+  });
+}
 
 template <int W>
 void probe_launch(const StepArgs& a, unsigned grid, bool one_query, hipStream_t s) {
@@ -1607,7 +1666,8 @@ void probe_launch(const StepArgs& a, unsigned grid, bool one_query, hipStream_t 
 }
 
 void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
-  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
+  const int wi = w == 8 ? 3 : w == 4 ? 2 : w == 2 ? 1 : 0;
+  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : g_grids.probe[wi];
   // w lanes share one key: each reads 8/w slots of the 128-B bucket, so a
   // wave's loads of one slot index cover 64/w buckets, each line coalesced
   switch (w) {
@@ -1623,7 +1683,7 @@ void launch_scan_tiles(const StepArgs& a, hipStream_t s) {
 }
 
 void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
-  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : kPersistentGrid;
+  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : g_grids.scan_rows;
   WK_LAUNCH(k_scan_rows, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
@@ -1633,7 +1693,9 @@ void launch_scan_queries(const StepArgs& a, hipStream_t s) {
 
 void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
   const uint32_t nqb = (a.nq + kBlock - 1) / kBlock;
-  WK_LAUNCH(k_query_sums, dim3(nqb + kPersistentGrid / 4), dim3(kBlock), 0, s, a, nqb);
+  // blocks past nqb do k_tile_rows' work
+  const unsigned extra = g_grids.query_sums / 4 > 64 ? g_grids.query_sums / 4 : 64;  // at least one
+  WK_LAUNCH(k_query_sums, dim3(nqb + extra), dim3(kBlock), 0, s, a, nqb);
   WK_LAUNCH(k_scan_queries_mb, dim3(nqb), dim3(kBlock), 0, s, a, nqb);
 }
 
@@ -1657,14 +1719,14 @@ void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
 
 void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s) {
   const uint64_t tiles = (out_hint + kOutTile - 1) / kOutTile;
-  const unsigned grid = out_hint ? grid_for((tiles + kBlock - 1) / kBlock, 1u << 20) : kPersistentGrid;
+  const unsigned grid = out_hint ? grid_for((tiles + kBlock - 1) / kBlock, 1u << 20) : g_grids.tile_rows;
   WK_LAUNCH(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
   if (rows_per_wave <= 0) {
     const unsigned grid =
-        out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : 4 * kPersistentGrid;
+        out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : g_grids.gather[a.ck ? 1 : 0];
     // the checksum variant only when final tables are checksummed (serving)
     if (a.ck)
       WK_LAUNCH(k_gather<true>, dim3(grid), dim3(kBlock), 0, s, a);
@@ -1673,12 +1735,11 @@ void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint,
     return;
   }
   // rows per wavefront -> lanes per row
-  const unsigned grid = kPersistentGrid * 2;
   switch (rows_per_wave) {
-    case 64: WK_LAUNCH(k_gather_rows<1>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 16: WK_LAUNCH(k_gather_rows<4>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 4: WK_LAUNCH(k_gather_rows<16>, dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: WK_LAUNCH(k_gather_rows<64>, dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 64: WK_LAUNCH(k_gather_rows<1>, dim3(g_grids.gather_rows[0]), dim3(kBlock), 0, s, a); break;
+    case 16: WK_LAUNCH(k_gather_rows<4>, dim3(g_grids.gather_rows[1]), dim3(kBlock), 0, s, a); break;
+    case 4: WK_LAUNCH(k_gather_rows<16>, dim3(g_grids.gather_rows[2]), dim3(kBlock), 0, s, a); break;
+    default: WK_LAUNCH(k_gather_rows<64>, dim3(g_grids.gather_rows[3]), dim3(kBlock), 0, s, a); break;
   }
 }
 

@@ -195,6 +195,10 @@ void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 // launch, `stop` from each (the last one's stays).  disarm_launch_timer
 // returns whether any launch took them.
 void arm_launch_timer(hipEvent_t start, hipEvent_t stop);
+// Sizes the grids of launches whose row counts live on the device from the
+// current device's CU count and each kernel's occupancy (once per process,
+// before any graph capture).
+void init_launch_grids();
 bool disarm_launch_timer();
 
 // Direct key index of a resident segment s (its buckets at karena through

@@ -442,7 +442,13 @@ int build_store(const uint32_t* pso, uint64_t n_pso, const uint32_t* pos, uint64
   for (auto& sb : builds) {
     rc = fill_segment(*sb, st->slots, st->values);
     if (rc) return rc;
-    st->seg_lookup[seg_code(sb->meta.id.index, sb->meta.id.pid, sb->meta.id.dir)] = int(st->segs.size());
+    const SegId& id = sb->meta.id;
+    st->seg_lookup[seg_code(id.index, id.pid, id.dir)] = int(st->segs.size());
+    if (id.index == 0 && id.dir < 2) {
+      const size_t i = 2 * size_t(id.pid) + id.dir;
+      if (st->seg_flat.size() <= i) st->seg_flat.resize(i + 1, -1);
+      st->seg_flat[i] = int32_t(st->segs.size());
+    }
     st->segs.push_back(sb->meta);
     sb.reset();
   }

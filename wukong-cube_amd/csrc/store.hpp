@@ -32,6 +32,7 @@ inline uint64_t seg_code(uint32_t index, uint32_t pid, uint32_t dir) {
 struct Store {
   std::vector<SegMeta> segs;
   std::unordered_map<uint64_t, int> seg_lookup;
+  std::vector<int32_t> seg_flat;  // normal segment of (pid, dir) at 2 * pid + dir, -1 if none
   Slot* slots = nullptr;       // 8 per bucket
   uint64_t total_buckets = 0;
   uint32_t* values = nullptr;
@@ -43,6 +44,10 @@ struct Store {
 
   ~Store();
   int seg_index(uint32_t index, uint32_t pid, uint32_t dir) const {
+    if (index == 0 && dir < 2) {
+      const std::size_t i = 2 * std::size_t(pid) + dir;
+      return i < seg_flat.size() ? seg_flat[i] : -1;
+    }
     auto it = seg_lookup.find(seg_code(index, pid, dir));
     return it == seg_lookup.end() ? -1 : it->second;
   }
