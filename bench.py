@@ -423,12 +423,18 @@ def main():
     rounds_all = []
 
     def run_slice(k):
-        """The whole job of slice k: all its queries, then those whose results
-        did not fit the rest of the result arena, in further rounds that reuse
-        the arena, until a round completes none (their results alone exceed
-        it).  Results stay on the device, as on the per-query path."""
-        base = k * args.nq
+        """The whole job of slice k, one call at a time: all its queries, then
+        those whose results did not fit the rest of the result arena, in
+        further rounds that reuse the arena.  Results stay on the device, as
+        on the per-query path."""
         r, _, st = dev.batch(slices[k], use_graph=not args.no_graph, fetch=False)
+        return more_rounds(k, r, st)
+
+    def more_rounds(k, r, st):
+        """Queries of slice k whose results did not fit the rest of the
+        result arena, in further synchronous rounds until a round completes
+        none (their results alone exceed it)."""
+        base = k * args.nq
         idx = np.arange(args.nq)
         nr = 1
         while True:
@@ -444,29 +450,62 @@ def main():
         st_all[base:base + args.nq] = st
         return nr
 
+    def run_pipelined(ks):
+        """Slices ks with two in flight: slice k + 1 is compiled, uploaded and
+        launched on the other lane while slice k runs on the GPU.  Returns the
+        rounds each slice took."""
+        pend, nr = [], []
+
+        def complete(k, tk):
+            dev.batch_wait(tk)
+            pb = slices[k]
+            nr.append(more_rounds(k, pb.rows.copy(), pb.status.copy()))
+
+        for k in ks:
+            pend.append((k, dev.batch_submit(slices[k], use_graph=not args.no_graph)))
+            if len(pend) == 2:
+                complete(*pend.pop(0))
+        for kt in pend:
+            complete(*kt)
+        return nr
+
     for k in range(args.warmup):
         run_slice(k)
     dev.sync()
+    # one slice after the other, each waited for before the next is prepared
     dev.reset_kernel_stats()
     t = time.perf_counter()
     for k in range(args.warmup, nslices):
         rounds_all.append(run_slice(k))
     dev.sync()
-    batch_s = (time.perf_counter() - t) / args.steps
-    timed = slice(args.warmup * args.nq, nslices * args.nq)
-    out["batch_rounds_max"] = int(max(rounds_all))
+    sync_s = (time.perf_counter() - t) / args.steps
     # host phases of wk_batch_run (the device's profile mode times them)
     batch_host = {k[5:]: v["total_ms"] / args.steps for k, v in dev.kernel_stats().items()
                   if k.startswith("host:")}
     out["batch_host_ms"] = batch_host
     if batch_host:
         log("batch host phases (ms per batch): " + ", ".join("%s %.3f" % kv for kv in sorted(batch_host.items())))
+    log("batched, one slice at a time: %.3f ms per %d-query slice -> %.0f q/s" % (
+        sync_s * 1e3, args.nq, args.nq / sync_s))
+    # the headline: the same slices pipelined, two in flight (value)
+    rows_all[:] = 0
+    st_all[:] = -1
+    run_pipelined(range(args.warmup))
+    dev.sync()
+    t = time.perf_counter()
+    rounds_all = run_pipelined(range(args.warmup, nslices))
+    dev.sync()
+    batch_s = (time.perf_counter() - t) / args.steps
+    out["batch_one_at_a_time"] = {"ms_per_step": sync_s * 1e3, "queries_per_s": args.nq / sync_s,
+                                  "note": "wk_batch_run per slice: host preparation and GPU run in sequence"}
+    timed = slice(args.warmup * args.nq, nslices * args.nq)
+    out["batch_rounds_max"] = int(max(rounds_all))
     ok_b = st_all[timed] == 0
     qps_batched = args.nq / batch_s
-    log("batched: %.3f ms per %d-query slice (up to %d rounds) -> %.0f q/s (%d ok, %d over the table budget, "
-        "%d result rows over %d slices)" % (batch_s * 1e3, args.nq, max(rounds_all), qps_batched, ok_b.sum(),
-                                           (st_all[timed] == wg.WK_E_NOMEM_BUDGET).sum(),
-                                           int(rows_all[timed][ok_b].sum()), args.steps))
+    log("batched, pipelined (two slices in flight): %.3f ms per %d-query slice (up to %d rounds) -> %.0f q/s "
+        "(%d ok, %d over the table budget, %d result rows over %d slices)" % (
+            batch_s * 1e3, args.nq, max(rounds_all), qps_batched, ok_b.sum(),
+            (st_all[timed] == wg.WK_E_NOMEM_BUDGET).sum(), int(rows_all[timed][ok_b].sum()), args.steps))
     assert ((st_all == 0) | (st_all == wg.WK_E_NOMEM_BUDGET)).all(), np.unique(st_all)
 
     # algorithmic bytes of the timed slices' pattern steps, from the device's
@@ -497,9 +536,12 @@ def main():
     host_buf = [np.zeros(1, np.uint32)]
     host_ptr = [None]
 
-    def fetch_arena():
+    def to_host(fetch):
+        """Copies a result arena to the pinned buffer through fetch(dst, cap,
+        used) (wk_batch_fetch of a ticket, or wk_batch_download); returns the
+        bytes copied."""
         used = wg.C.c_uint64()
-        wg._check(L.wk_batch_download(dev._h, None, 0, wg.C.byref(used)))
+        wg._check(fetch(None, 0, wg.C.byref(used)))
         if host_buf[0].size < used.value:
             if host_ptr[0]:
                 hip.hipHostFree(host_ptr[0])
@@ -508,37 +550,49 @@ def main():
             assert hip.hipHostMalloc(wg.C.byref(ptr), wg.C.c_size_t(n * 4), 0) == 0
             host_ptr[0] = ptr
             host_buf[0] = np.ctypeslib.as_array(wg.C.cast(ptr, wg.U32P), (n,))
-        wg._check(L.wk_batch_download(dev._h, host_buf[0].ctypes.data_as(wg.U32P), host_buf[0].size,
-                                      wg.C.byref(used)))
+        wg._check(fetch(host_buf[0].ctypes.data_as(wg.U32P), host_buf[0].size, wg.C.byref(used)))
         return used.value * 4
 
-    def run_slice_fetch(k):
-        base = k * args.nq
-        nbytes = 0
-        _, _, st = dev.batch(slices[k], use_graph=not args.no_graph, fetch=False)
-        nbytes += fetch_arena()
-        idx = np.arange(args.nq)
-        while True:
-            fail = idx[st[idx] == wg.WK_E_NOMEM_BUDGET]
-            if len(fail) == 0 or len(fail) == len(idx):
-                break
-            _, _, s2 = dev.batch(wg.PreparedBatch([qs[base + i] for i in fail]), use_graph=not args.no_graph,
-                                 fetch=False)
-            nbytes += fetch_arena()
-            st[fail] = s2
-            idx = fail
-        return nbytes
+    def run_fetch_pipelined(ks):
+        """run_pipelined with each slice's arena copied to the host after its
+        wait (while the next slice runs), and the arenas of further rounds
+        after each round."""
+        pend, nbytes = [], [0]
 
-    run_slice_fetch(0)
+        def complete(k, tk):
+            dev.batch_wait(tk)
+            nbytes[0] += to_host(lambda d, c, u: L.wk_batch_fetch(dev._h, tk, d, c, u))
+            base = k * args.nq
+            st = slices[k].status.copy()
+            idx = np.arange(args.nq)
+            while True:
+                fail = idx[st[idx] == wg.WK_E_NOMEM_BUDGET]
+                if len(fail) == 0 or len(fail) == len(idx):
+                    break
+                _, _, s2 = dev.batch(wg.PreparedBatch([qs[base + i] for i in fail]),
+                                     use_graph=not args.no_graph, fetch=False)
+                nbytes[0] += to_host(lambda d, c, u: L.wk_batch_download(dev._h, d, c, u))
+                st[fail] = s2
+                idx = fail
+
+        for k in ks:
+            pend.append((k, dev.batch_submit(slices[k], use_graph=not args.no_graph)))
+            if len(pend) == 2:
+                complete(*pend.pop(0))
+        for kt in pend:
+            complete(*kt)
+        return nbytes[0]
+
+    run_fetch_pipelined(range(args.warmup))
+    dev.sync()
     t = time.perf_counter()
-    fetched = 0
-    for k in range(args.warmup, nslices):
-        fetched += run_slice_fetch(k)
+    fetched = run_fetch_pipelined(range(args.warmup, nslices))
     dev.sync()
     fetch_s = (time.perf_counter() - t) / args.steps
     out["pcie_inclusive"] = {"queries_per_s": args.nq / fetch_s, "ms_per_step": fetch_s * 1e3,
                              "result_bytes_per_step": fetched / args.steps,
-                             "note": "each round's result arena copied to pinned host memory"}
+                             "note": "pipelined as the headline, each slice's result arena copied to pinned "
+                                     "host memory after its wait, while the next slice runs"}
     log("batched with results copied to the host: %.3f ms per slice (%.1f MB) -> %.0f q/s"
         % (fetch_s * 1e3, fetched / args.steps / 1e6, args.nq / fetch_s))
     host_buf[0] = None

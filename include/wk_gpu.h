@@ -320,6 +320,25 @@ typedef struct {
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph,
                  wk_batch_result* out);
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems);
+/* Pipelined batches (no counterpart in the reference, whose agent runs one
+ * query at a time): wk_batch_submit compiles, uploads and launches a batch
+ * as wk_batch_run does and returns without waiting for the GPU, so that the
+ * caller can prepare and submit the next batch while this one runs.  Two
+ * batches can be in flight at once, each on its own lane (stream, history
+ * tables, scratch and result arena, allocated with the handle's table budget
+ * on first use of the second lane); a third submit returns WK_E_STATE.
+ * `out` must stay valid until the batch is waited for: cols and v2c are
+ * written by the submit, rows, status and offset by wk_batch_wait, or by any
+ * other call on the handle that uses a lane, which first completes the
+ * batches in flight.  wk_batch_fetch copies a waited-for ticket's result
+ * arena (offsets as in `out`) to the host until its lane starts another
+ * batch or another call uses the lanes (WK_E_STATE after that).  Under a
+ * gstore budget smaller than the store, a submit first completes the batch
+ * on the other lane, so that its pinned segments cannot starve this one. */
+int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out,
+                    uint32_t* ticket);
+int wk_batch_wait(wk_dev* dev, uint32_t ticket);
+int wk_batch_fetch(wk_dev* dev, uint32_t ticket, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems);
 /* Measurement (no counterpart in the reference).  With enable != 0, each
  * later wk_batch_run reads its pattern steps' descriptors back once it has
  * completed and adds their algorithmic bytes to a running total, by the

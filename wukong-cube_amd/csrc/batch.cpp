@@ -97,11 +97,14 @@ int run_alone(wk_dev* dev, const Plan& plan, uint64_t arena_base, uint64_t* rows
   return WK_OK;
 }
 
-int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
-              bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end) {
+int begin_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+                bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, BatchLaunch& bl) {
   const uint32_t nq = q1 - q0;
   double t = host_now();
-  *arena_end = arena_base;
+  bl.running = false;
+  bl.S = 0;
+  bl.arena_base = bl.arena_end = arena_base;
+  bl.segs.clear();
   uint32_t S = 0;
   for (uint32_t q = q0; q < q1; q++) {
     rows[q] = 0;
@@ -159,22 +162,27 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   }
   dev->h_rs[nq] = nvalid;
   t = host_phase(dev, "host:batch_fill", t);
-  std::vector<int> segs;
+  std::vector<int>& segs = bl.segs;
   for (size_t i = 0; i < seen.size(); i++)
     if (seen[i]) segs.push_back(int(i));
   rc = ensure_segments(dev, segs);
   if (rc == WK_E_NOMEM_BUDGET || rc == WK_E_SEG_TOO_LARGE) {
     // the segments of all the batch's steps do not fit the gstore budget
     // together: run the halves one after the other, as for a table overflow
-    if (nq > 1) return split_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, arena_end);
-    return run_alone(dev, plans[q0], arena_base, &rows[q0], &offsets[q0], &status[q0], arena_end);
+    segs.clear();
+    if (nq > 1) return split_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, &bl.arena_end);
+    return run_alone(dev, plans[q0], arena_base, &rows[q0], &offsets[q0], &status[q0], &bl.arena_end);
   }
-  if (rc) return rc;
+  if (rc) {
+    segs.clear();
+    return rc;
+  }
   t = host_phase(dev, "host:batch_segs", t);
   for (uint32_t k = 0; k <= S; k++) dev->h_state[k] = StepState{};
   dev->h_state[0].arena_base = arena_base;
   auto fail = [&](int code) {
     unpin_segments(dev, segs);
+    segs.clear();
     return code;
   };
   // steps 0..S-1 are read; the last step writes no next-step descriptors
@@ -208,14 +216,29 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   } else {
     launch_batch_steps(dev, 0, S, S, nq, dev->opts.profile != 0);
   }
-  t = host_phase(dev, "host:batch_upload_launch", t);
+  // the per-query results and step states come back behind the steps
   if (hip_failed(hipMemcpyAsync(dev->h_qres, dev->d_qres, size_t(nq) * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                 dev->stream)) ||
-      hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
-      hip_failed(hipStreamSynchronize(dev->stream)))
+      hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, S * sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)))
     return fail(WK_E_HIP);
-  unpin_segments(dev, segs);
-  t = host_phase(dev, "host:batch_wait", t);
+  host_phase(dev, "host:batch_upload_launch", t);
+  bl.S = S;
+  bl.running = true;
+  return WK_OK;
+}
+
+int finish_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, bool use_graph,
+                 uint64_t* rows, uint64_t* offsets, int32_t* status, BatchLaunch& bl) {
+  if (!bl.running) return WK_OK;
+  bl.running = false;
+  const uint32_t nq = q1 - q0, S = bl.S;
+  const uint64_t arena_base = bl.arena_base;
+  double t = host_now();
+  const bool ok = !hip_failed(hipStreamSynchronize(dev->stream));
+  unpin_segments(dev, bl.segs);
+  bl.segs.clear();
+  if (!ok) return WK_E_HIP;
+  host_phase(dev, "host:batch_wait", t);
   if (dev->traffic_on) {
     int trc = count_traffic(dev, S, nq);
     if (trc) return trc;
@@ -225,8 +248,8 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   if (overflow) {
     // an intermediate table overflowed: split the batch; halves reuse the
     // tables and append to the arena
-    if (nq == 1) return run_alone(dev, plans[q0], arena_base, &rows[q0], &offsets[q0], &status[q0], arena_end);
-    return split_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, arena_end);
+    if (nq == 1) return run_alone(dev, plans[q0], arena_base, &rows[q0], &offsets[q0], &status[q0], &bl.arena_end);
+    return split_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, &bl.arena_end);
   }
   uint64_t end = arena_base;
   for (uint32_t i = 0; i < nq; i++) {
@@ -242,9 +265,18 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
     offsets[q] = dev->h_qres[2 * size_t(i) + 1];
     end = std::max<uint64_t>(end, offsets[q] + rows[q] * p.cols);
   }
-  *arena_end = end;
+  bl.arena_end = end;
   dev->stats.queries_run += nq;
   return WK_OK;
+}
+
+int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+              bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end) {
+  BatchLaunch bl;
+  int rc = begin_batch(dev, plans, q0, q1, arena_base, use_graph, rows, offsets, status, bl);
+  if (!rc) rc = finish_batch(dev, plans, q0, q1, use_graph, rows, offsets, status, bl);
+  *arena_end = bl.arena_end;
+  return rc;
 }
 
 }  // namespace wk

@@ -258,6 +258,7 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   nv = std::max<uint64_t>(nv, 1);
   d->nkblk = uint32_t(nk);
   d->nvblk = uint32_t(nv);
+  d->swapping = nk < kt || nv < vt;
   WK_HIP(hipMalloc(&d->karena, nk * kbytes));
   WK_HIP(hipMalloc(&d->varena, nv * vbytes));
   if (!st->pinned) {
@@ -283,6 +284,86 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
 }
 
 }  // namespace
+
+namespace {
+
+// Plans of a batch into the current lane's plan storage (kept from call to
+// call), with each query's column count and var->column map written out.
+void compile_batch(wk_dev* dev, const wk_query* qs, uint32_t nq, const wk_batch_result* out) {
+  const double t0 = host_now();
+  std::vector<Plan>& plans = dev->plan_cache;
+  if (plans.size() < nq) plans.resize(nq);
+#pragma omp parallel for schedule(static) if (nq >= 1024)
+  for (uint32_t i = 0; i < nq; i++) {
+    compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
+    out->cols[i] = plans[i].cols;
+    if (out->v2c) {
+      // the query's own variables only (a full row of WK_MAX_VARS entries
+      // per query was a megabyte of host writes per 4,096-query batch)
+      const int32_t nv = std::min<int32_t>(std::max<int32_t>(qs[i].nvars, 0), WK_MAX_VARS);
+      int32_t* m = out->v2c + size_t(i) * WK_MAX_VARS;
+      for (int32_t v = 0; v < nv; v++) m[v] = plans[i].status == WK_OK ? plans[i].v2c[size_t(v)] : -1;
+    }
+  }
+  host_phase(dev, "host:batch_compile", t0);
+}
+
+bool valid_result(const wk_batch_result* out) {
+  return out && out->rows && out->cols && out->status && out->offset;
+}
+
+// Completes the current lane's submitted batch: waits for its steps and
+// fills the caller's result arrays.
+int finish_lane(wk_dev* dev) {
+  if (!dev->inflight) return WK_OK;
+  dev->inflight = false;
+  const wk_batch_result& o = dev->b_out;
+  const int rc = finish_batch(dev, dev->plan_cache, 0, dev->b_nq, dev->b_graph, o.rows, o.offset, o.status,
+                              dev->launch);
+  dev->batch_used = rc ? 0 : dev->launch.arena_end;
+  dev->fetchable = rc == WK_OK;
+  return rc;
+}
+
+// Lane 0 is the device's own, lane 1 the other one (swapped in around the
+// call).
+template <typename F>
+int on_lane(wk_dev* dev, int lane, F f) {
+  if (lane == 0) return f();
+  swap_lanes(dev);
+  const int rc = f();
+  swap_lanes(dev);
+  return rc;
+}
+
+}  // namespace
+
+namespace wk {
+
+int ensure_alt_lane(wk_dev* dev) {
+  if (dev->alt.stream) return WK_OK;
+  swap_lanes(dev);
+  int rc = WK_OK;
+  if (hip_failed(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking))) rc = WK_E_HIP;
+  if (!rc) rc = alloc_lane(dev, dev->opts.table_budget);
+  if (rc) free_lane(*dev);
+  swap_lanes(dev);
+  return rc;
+}
+
+int drain_batches(wk_dev* dev) {
+  int rc = WK_OK;
+  if (dev->inflight) rc = finish_lane(dev);
+  if (dev->alt.inflight) {
+    const int r2 = on_lane(dev, 1, [&]() -> int { return finish_lane(dev); });
+    if (!rc) rc = r2;
+  }
+  // the call that drains may reuse the arenas
+  dev->fetchable = dev->alt.fetchable = false;
+  return rc;
+}
+
+}  // namespace wk
 
 extern "C" {
 
@@ -440,6 +521,7 @@ int wk_dev_set_profile(wk_dev* dev, int32_t on) {
 
 int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
   if (!dev) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   if (probe_width != 1 && probe_width != 2 && probe_width != 4 && probe_width != 8) return WK_E_INVALID_ARG;
   if (rows_per_wave != 0 && rows_per_wave != 1 && rows_per_wave != 4 && rows_per_wave != 16 &&
       rows_per_wave != 64)
@@ -459,6 +541,7 @@ int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
 
 int wk_dev_sync(wk_dev* dev) {
   if (!dev) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   WK_HIP(hipStreamSynchronize(dev->copy_stream));
   WK_HIP(hipStreamSynchronize(dev->stream));
   return WK_OK;
@@ -466,6 +549,7 @@ int wk_dev_sync(wk_dev* dev) {
 
 int wk_dev_load_all(wk_dev* dev) {
   if (!dev) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   std::vector<int> none;
   for (size_t i = 0; i < dev->seg.size(); i++) {
     std::vector<int> one{int(i)};
@@ -479,6 +563,7 @@ int wk_dev_load_all(wk_dev* dev) {
 
 int wk_seg_ensure(wk_dev* dev, uint32_t index, uint32_t pid, int32_t dir) {
   if (!dev || (dir != WK_IN && dir != WK_OUT)) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   const int s = dev->st->seg_index(index, pid, uint32_t(dir));
   if (s < 0) return WK_E_INVALID_ARG;
   std::vector<int> one{s};
@@ -545,6 +630,7 @@ int wk_table_free(wk_dev* dev, wk_table* t) {
 int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t nparts, wk_table** out,
                        uint64_t* sizes, uint64_t* heads) {
   if (!dev || !in || !out || !sizes || !heads) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   *out = nullptr;
   if (nparts == 0 || nparts > kMaxParts || (in->rows && col >= in->cols)) return WK_E_INVALID_ARG;
   const uint64_t rows = in->rows;
@@ -586,6 +672,7 @@ int wk_table_partition(wk_dev* dev, const wk_table* in, uint32_t col, uint32_t n
 // probed again (unless it is the only one) and gathered at its offset.
 static int pattern_step(wk_dev* dev, StepDesc sd, const wk_table* in, wk_table** out) {
   if (!dev || !out) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   *out = nullptr;
   const uint64_t R = in ? in->rows : sd.in_rows;
   const uint32_t* tin = in ? in->d : nullptr;
@@ -731,6 +818,7 @@ int wk_expand_c2k(wk_dev* dev, const wk_table* in, uint32_t vid, uint32_t pid, i
 
 int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols, int32_t* v2c) {
   if (!dev || !q) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   dev->last_ptr = nullptr;
   dev->last_rows = 0;
   dev->last_cols = 0;
@@ -767,35 +855,94 @@ int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems) {
 
 // ------------------------------------------------------------------ batch
 
+
 int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out) {
-  if (!dev || (!qs && nq) || !out || !out->rows || !out->cols || !out->status || !out->offset)
-    return WK_E_INVALID_ARG;
-  const double t0 = host_now();
-  // plans keep their storage from call to call
-  std::vector<Plan>& plans = dev->plan_cache;
-  if (plans.size() < nq) plans.resize(nq);
-#pragma omp parallel for schedule(static) if (nq >= 1024)
-  for (uint32_t i = 0; i < nq; i++) {
-    compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
-    out->cols[i] = plans[i].cols;
-    if (out->v2c) {
-      // the query's own variables only (a full row of WK_MAX_VARS entries
-      // per query was a megabyte of host writes per 4,096-query batch)
-      const int32_t nv = std::min<int32_t>(std::max<int32_t>(qs[i].nvars, 0), WK_MAX_VARS);
-      int32_t* m = out->v2c + size_t(i) * WK_MAX_VARS;
-      for (int32_t v = 0; v < nv; v++) m[v] = plans[i].status == WK_OK ? plans[i].v2c[size_t(v)] : -1;
-    }
-  }
-  host_phase(dev, "host:batch_compile", t0);
+  if (!dev || (!qs && nq) || !valid_result(out)) return WK_E_INVALID_ARG;
+  int rc = drain_batches(dev);
+  if (rc) return rc;
+  compile_batch(dev, qs, nq, out);
   uint64_t end = 0;
-  int rc = run_batch(dev, plans, 0, nq, 0, use_graph != 0, out->rows, out->offset, out->status, &end);
+  rc = run_batch(dev, dev->plan_cache, 0, nq, 0, use_graph != 0, out->rows, out->offset, out->status, &end);
   dev->batch_used = rc ? 0 : end;
+  dev->ticket = 0;
   return rc;
+}
+
+int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out,
+                    uint32_t* ticket) {
+  if (!dev || (!qs && nq) || !valid_result(out) || !ticket) return WK_E_INVALID_ARG;
+  *ticket = 0;
+  // the preferred lane if it is free, else the other; both busy: wait first
+  int lane = dev->next_lane;
+  if ((lane == 0 ? dev->inflight : dev->alt.inflight)) lane ^= 1;
+  if ((lane == 0 ? dev->inflight : dev->alt.inflight)) return WK_E_STATE;
+  if (lane == 1) {
+    const int rc = ensure_alt_lane(dev);
+    if (rc) return rc;
+  }
+  if (dev->swapping && (lane == 0 ? dev->alt.inflight : dev->inflight)) {
+    // under a gstore budget smaller than the store, the other lane's pinned
+    // segments could leave this batch no room: finish that one first
+    const int rc = on_lane(dev, lane ^ 1, [&]() -> int { return finish_lane(dev); });
+    if (rc) return rc;
+  }
+  const uint32_t t = dev->next_ticket++;
+  if (dev->next_ticket == 0) dev->next_ticket = 1;
+  const int rc = on_lane(dev, lane, [&]() -> int {
+    compile_batch(dev, qs, nq, out);
+    dev->ticket = t;
+    dev->fetchable = false;
+    dev->batch_used = 0;
+    dev->b_out = *out;
+    dev->b_nq = nq;
+    dev->b_graph = use_graph != 0;
+    const int r = begin_batch(dev, dev->plan_cache, 0, nq, 0, dev->b_graph, out->rows, out->offset, out->status,
+                              dev->launch);
+    if (r) {
+      dev->ticket = 0;
+      return r;
+    }
+    // a batch that already completed (no steps, or split for lack of
+    // gstore room) is finished by its wait like any other
+    dev->inflight = true;
+    return WK_OK;
+  });
+  if (rc) return rc;
+  dev->next_lane = lane ^ 1;
+  *ticket = t;
+  return WK_OK;
+}
+
+int wk_batch_wait(wk_dev* dev, uint32_t ticket) {
+  if (!dev || ticket == 0) return WK_E_INVALID_ARG;
+  for (int lane = 0; lane < 2; lane++) {
+    LaneState& l = lane == 0 ? static_cast<LaneState&>(*dev) : dev->alt;
+    if (l.ticket != ticket) continue;
+    if (!l.inflight) return WK_OK;  // already finished
+    return on_lane(dev, lane, [&]() -> int { return finish_lane(dev); });
+  }
+  return WK_E_STATE;
+}
+
+int wk_batch_fetch(wk_dev* dev, uint32_t ticket, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {
+  if (!dev || ticket == 0) return WK_E_INVALID_ARG;
+  for (int lane = 0; lane < 2; lane++) {
+    LaneState& l = lane == 0 ? static_cast<LaneState&>(*dev) : dev->alt;
+    if (l.ticket != ticket) continue;
+    if (l.inflight || !l.fetchable) return WK_E_STATE;  // not waited for, or its arena reused since
+    if (used_elems) *used_elems = l.batch_used;
+    if (!dst) return WK_OK;
+    if (cap_elems < l.batch_used) return WK_E_INVALID_ARG;
+    if (l.batch_used) WK_HIP(hipMemcpy(dst, l.arena, l.batch_used * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return WK_OK;
+  }
+  return WK_E_STATE;
 }
 
 int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, int32_t* probe_width,
                     int32_t* rows_per_wave, double* batch_ms) {
   if (!dev || (!qs && nq)) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   std::vector<Plan>& plans = dev->plan_cache;
   if (plans.size() < nq) plans.resize(nq);
 #pragma omp parallel for schedule(static) if (nq >= 1024)
@@ -851,6 +998,7 @@ int wk_batch_traffic(wk_dev* dev, int32_t enable, double* alg_bytes, double* edg
 
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {
   if (!dev) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   if (used_elems) *used_elems = dev->batch_used;
   if (!dst) return WK_OK;
   if (cap_elems < dev->batch_used) return WK_E_INVALID_ARG;

@@ -65,6 +65,18 @@ struct KernelStat {
   double bytes = 0;
 };
 
+// What begin_batch left on the lane's stream for finish_batch: the pattern
+// steps launched and the segments pinned for them.  `running` is false when
+// nothing is left to wait for (no steps, or the batch already completed by
+// itself, split or alone).
+struct BatchLaunch {
+  bool running = false;
+  uint32_t S = 0;
+  uint64_t arena_base = 0;
+  uint64_t arena_end = 0;
+  std::vector<int> segs;
+};
+
 struct PendingTiming {
   std::string name;
   hipEvent_t a, b;
@@ -123,6 +135,23 @@ struct LaneState {
   // chunk boundaries of a step run in output windows: [kWinGroup][3]
   uint64_t* d_win = nullptr;
   uint64_t* h_win = nullptr;  // pinned mirror
+
+  // compiled plans of the lane's last batch (storage reused by the next)
+  std::vector<Plan> plan_cache;
+  // elements of the arena holding the last batch's results
+  uint64_t batch_used = 0;
+
+  // A batch submitted on this lane (wk_batch_submit) whose GPU work may
+  // still run: its launch state and the caller's result arrays, which
+  // finish_batch fills.  `ticket` stays the ticket of the batch whose
+  // results the arena holds until the lane's next batch.
+  bool inflight = false;
+  bool fetchable = false;  // the arena still holds that batch's tables
+  uint32_t ticket = 0;
+  uint32_t b_nq = 0;
+  bool b_graph = false;
+  BatchLaunch launch;
+  wk_batch_result b_out{};
 };
 
 }  // namespace wk
@@ -139,6 +168,7 @@ struct wk_dev : wk::LaneState {
   // gstore cache
   uint32_t kshift = 14, vshift = 19;
   uint32_t nkblk = 0, nvblk = 0;
+  bool swapping = false;  // the gstore budget holds fewer blocks than the store
   wk::Slot* karena = nullptr;
   uint32_t* varena = nullptr;
   std::vector<uint32_t> free_k, free_v;
@@ -172,15 +202,14 @@ struct wk_dev : wk::LaneState {
   // the store itself is pageable; they are read in place by the kernels
   std::vector<void*> mapped_host;
 
-  // compiled plans of the last batch (storage reused by the next)
-  std::vector<wk::Plan> plan_cache;
-
   // last per-query result (in the result arena)
   const uint32_t* last_ptr = nullptr;
   uint64_t last_rows = 0;
   uint32_t last_cols = 0;
-  // last batch
-  uint64_t batch_used = 0;
+  // tickets of pipelined batches (wk_batch_submit), and the lane the next
+  // one prefers (0: this lane, 1: the other)
+  uint32_t next_ticket = 1;
+  int next_lane = 0;
   // batched-step traffic accounting (wk_batch_traffic)
   bool traffic_on = false;
   double traffic_bytes = 0, traffic_edges = 0;
@@ -255,8 +284,25 @@ int gather_window(wk_dev* dev, const StepDesc& sd, const uint32_t* tin, uint64_t
 // pool.
 int run_query_to(wk_dev* dev, const Plan& plan, uint32_t* dst, uint64_t cap, uint64_t* rows);
 
-// Batched execution of queries [q0, q1) of `plans`; results land in the
-// arena from `arena_base`.  Fills rows/offsets/status per query.
+// The other lane's stream, tables and scratch (same table budget), made on
+// first use.
+int ensure_alt_lane(wk_dev* dev);
+// Completes every batch submitted with wk_batch_submit and not yet waited
+// for (their result arrays are filled); every entry point that uses a lane
+// calls it first.
+int drain_batches(wk_dev* dev);
+
+// Batched execution of queries [q0, q1) of `plans`, in two halves:
+// begin_batch fills and uploads the step descriptors, makes the segments
+// resident and launches every pattern step on the lane's stream without
+// waiting; finish_batch waits for them, releases the segments and fills
+// rows/offsets/status per query, rerunning the batch in halves when a step
+// overflowed.  Results land in the arena from `arena_base`.
+int begin_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
+                bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, BatchLaunch& bl);
+int finish_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, bool use_graph,
+                 uint64_t* rows, uint64_t* offsets, int32_t* status, BatchLaunch& bl);
+// begin_batch then finish_batch.
 int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
               bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end);
 // Queries [q0, q1) as two batches, one after the other.

@@ -48,6 +48,7 @@ extern "C" int wk_table_final_process(wk_dev* dev, const wk_table* in, const int
   if (!dev || !in || !out || nvars < 0 || (nvars && !v2c) || (nreq && !required) ||
       (norder && (!order_vars || !order_desc)))
     return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   *out = nullptr;
   const uint64_t rows = in->rows;
   const uint32_t cols = in->cols;

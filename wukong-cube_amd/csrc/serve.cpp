@@ -223,14 +223,7 @@ struct HeavyLane {
   int ready() {
     if (!done) WK_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     if (!seeded) WK_HIP(hipEventCreateWithFlags(&seeded, hipEventDisableTiming));
-    if (dev->alt.stream) return WK_OK;
-    swap_lanes(dev);
-    int rc = WK_OK;
-    if (hip_failed(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking))) rc = WK_E_HIP;
-    if (!rc) rc = alloc_lane(dev, dev->opts.table_budget);
-    if (rc) free_lane(*dev);
-    swap_lanes(dev);
-    return rc;
+    return ensure_alt_lane(dev);
   }
 
   // Elements of the lane's first input table (the continued tables).
@@ -351,6 +344,7 @@ using namespace wk;
 extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const double* arrival_s, uint32_t nq,
                                const wk_serve_opts* opts, wk_serve_result* out) {
   if (!dev || (!queries && nq) || (!arrival_s && nq) || !opts || !out) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   for (uint32_t i = 1; i < nq; i++)
     if (arrival_s[i] < arrival_s[i - 1]) return WK_E_INVALID_ARG;
   std::vector<Plan> plans(nq);

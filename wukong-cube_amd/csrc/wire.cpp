@@ -393,6 +393,7 @@ void wk_wire_free(wk_sparql_msg* m) { delete reinterpret_cast<Holder*>(m); }
 
 int wk_agent_execute(wk_dev* dev, const uint8_t* req, uint64_t len, const wk_wire_opts* o, uint64_t* reply_len) {
   if (!dev || !reply_len || (len && !req)) return WK_E_INVALID_ARG;
+  if (int drc = drain_batches(dev)) return drc;
   *reply_len = 0;
   dev->agent_reply.clear();
   Holder q;

@@ -183,6 +183,10 @@ SIGNATURES = {
     "wk_dev_autotune": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "wk_batch_download": (C.c_int, [P, U32P, C.c_uint64, U64P]),
+    "wk_batch_submit": (C.c_int, [P, C.POINTER(Query), C.c_uint32, C.c_int32, C.POINTER(BatchResult),
+                                  C.POINTER(C.c_uint32)]),
+    "wk_batch_wait": (C.c_int, [P, C.c_uint32]),
+    "wk_batch_fetch": (C.c_int, [P, C.c_uint32, U32P, C.c_uint64, U64P]),
     "wk_batch_traffic": (C.c_int, [P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), U64P]),
     "wk_serve_replay": (C.c_int, [P, C.POINTER(Query), C.POINTER(C.c_double), C.c_uint32,
                                   C.POINTER(ServeOpts), C.POINTER(ServeResult)]),
@@ -671,11 +675,38 @@ class Device:
         _check(lib().wk_batch_download(self._h, None, 0, C.byref(used)))
         arena = np.zeros(max(1, used.value), np.uint32)
         _check(lib().wk_batch_download(self._h, arena.ctypes.data_as(U32P), arena.size, C.byref(used)))
+        return self._tables(pb, arena)
+
+    def batch_submit(self, pb, use_graph=True, v2c=False):
+        """Pipelined batch: launches the PreparedBatch `pb` and returns its
+        ticket without waiting for the GPU (two may be in flight).  pb's result
+        arrays are filled by batch_wait (or by any later call that completes
+        it); pb must not be submitted again before then."""
+        t = C.c_uint32()
+        res = pb.res if v2c else pb.res_rows_only
+        _check(lib().wk_batch_submit(self._h, pb.qs, pb.n, 1 if use_graph else 0, C.byref(res), C.byref(t)),
+               "wk_batch_submit")
+        return t.value
+
+    def batch_wait(self, ticket):
+        _check(lib().wk_batch_wait(self._h, ticket), "wk_batch_wait")
+
+    def batch_fetch(self, ticket, pb):
+        """The tables of a waited-for ticket (submitted with v2c=True), as
+        Device.batch returns them."""
+        used = C.c_uint64()
+        _check(lib().wk_batch_fetch(self._h, ticket, None, 0, C.byref(used)), "wk_batch_fetch")
+        arena = np.zeros(max(1, used.value), np.uint32)
+        _check(lib().wk_batch_fetch(self._h, ticket, arena.ctypes.data_as(U32P), arena.size, C.byref(used)),
+               "wk_batch_fetch")
+        return self._tables(pb, arena)
+
+    def _tables(self, pb, arena):
         out = []
         for i in range(pb.n):
-            r, c, o = int(rows[i]), int(cols[i]), int(offs[i])
+            r, c, o = int(pb.rows[i]), int(pb.cols[i]), int(pb.offs[i])
             t = arena[o:o + r * c].reshape(r, c) if c else np.zeros((0, 0), np.uint32)
-            out.append((t.copy(), list(v2c[i * MAX_VARS:i * MAX_VARS + pb.nvars[i]]), int(status[i])))
+            out.append((t.copy(), list(pb.v2c[i * MAX_VARS:i * MAX_VARS + pb.nvars[i]]), int(pb.status[i])))
         return out
 
     def batch_traffic(self, enable):
