@@ -96,6 +96,8 @@ def parse_args():
     ap.add_argument("--dense-index", type=int, default=0,
                     help="direct key indexes of dense segments: 0 = build (default), -1 = probe the buckets only")
     ap.add_argument("--no-graph", action="store_true", help="launch batched steps directly (no hipGraph)")
+    ap.add_argument("--batch-lanes", type=int, default=0,
+                    help="lanes the pipelined batches rotate over, 1..4 (0 = library default, 2)")
     ap.add_argument("--single-only", action="store_true",
                     help="only the single-query kernels and their roofline (the launches a kernel trace "
                          "of this command should cover)")
@@ -370,7 +372,8 @@ def main():
     log("host gstore built in %.1fs: %s" % (time.time() - t0, info))
     dev = wg.Device(store, gstore_budget=int(args.gstore_budget_gb * (1 << 30)),
                     table_budget=int(args.table_budget_gb * (1 << 30)), profile=1,
-                    probe_width=args.probe_width, rows_per_wave=args.rows_per_wave, dense_index=args.dense_index)
+                    probe_width=args.probe_width, rows_per_wave=args.rows_per_wave, dense_index=args.dense_index,
+                    batch_lanes=args.batch_lanes)
     t0 = time.time()
     dev.load_all()
     load_s = time.time() - t0
@@ -450,10 +453,12 @@ def main():
         st_all[base:base + args.nq] = st
         return nr
 
+    inflight = args.batch_lanes or 2  # slices in flight: one per lane
+
     def run_pipelined(ks):
-        """Slices ks with two in flight: slice k + 1 is compiled, uploaded and
-        launched on the other lane while slice k runs on the GPU.  Returns the
-        rounds each slice took."""
+        """Slices ks with one in flight per lane: slice k + 1 is compiled,
+        uploaded and launched on the next lane while slice k runs on the GPU.
+        Returns the rounds each slice took."""
         pend, nr = [], []
 
         def complete(k, tk):
@@ -463,7 +468,7 @@ def main():
 
         for k in ks:
             pend.append((k, dev.batch_submit(slices[k], use_graph=not args.no_graph)))
-            if len(pend) == 2:
+            if len(pend) == inflight:
                 complete(*pend.pop(0))
         for kt in pend:
             complete(*kt)
@@ -502,9 +507,9 @@ def main():
     out["batch_rounds_max"] = int(max(rounds_all))
     ok_b = st_all[timed] == 0
     qps_batched = args.nq / batch_s
-    log("batched, pipelined (two slices in flight): %.3f ms per %d-query slice (up to %d rounds) -> %.0f q/s "
+    log("batched, pipelined (%d slices in flight): %.3f ms per %d-query slice (up to %d rounds) -> %.0f q/s "
         "(%d ok, %d over the table budget, %d result rows over %d slices)" % (
-            batch_s * 1e3, args.nq, max(rounds_all), qps_batched, ok_b.sum(),
+            inflight, batch_s * 1e3, args.nq, max(rounds_all), qps_batched, ok_b.sum(),
             (st_all[timed] == wg.WK_E_NOMEM_BUDGET).sum(), int(rows_all[timed][ok_b].sum()), args.steps))
     assert ((st_all == 0) | (st_all == wg.WK_E_NOMEM_BUDGET)).all(), np.unique(st_all)
 
@@ -577,7 +582,7 @@ def main():
 
         for k in ks:
             pend.append((k, dev.batch_submit(slices[k], use_graph=not args.no_graph)))
-            if len(pend) == 2:
+            if len(pend) == inflight:
                 complete(*pend.pop(0))
         for kt in pend:
             complete(*kt)

@@ -176,6 +176,8 @@ typedef struct {
                                  ptr word per vid, probed instead of the buckets): 0 = build
                                  them within a quarter of free HBM (and of gstore_budget when
                                  set), most keys first; -1 = never */
+  int32_t batch_lanes;        /* lanes pipelined batches rotate over (wk_batch_submit), each
+                                 with its own stream, tables and arena: 1..4 (0 -> 2) */
 } wk_dev_opts;
 
 typedef struct {
@@ -324,9 +326,10 @@ int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* 
  * query at a time): wk_batch_submit compiles, uploads and launches a batch
  * as wk_batch_run does and returns without waiting for the GPU, so that the
  * caller can prepare and submit the next batch while this one runs.  Two
- * batches can be in flight at once, each on its own lane (stream, history
- * tables, scratch and result arena, allocated with the handle's table budget
- * on first use of the second lane); a third submit returns WK_E_STATE.
+ * opts.batch_lanes batches (2 by default) can be in flight at once, each on
+ * its own lane (stream, history tables, scratch and result arena, allocated
+ * with the handle's table budget on a lane's first use); one more submit
+ * returns WK_E_STATE.
  * `out` must stay valid until the batch is waited for: cols and v2c are
  * written by the submit, rows, status and offset by wk_batch_wait, or by any
  * other call on the handle that uses a lane, which first completes the

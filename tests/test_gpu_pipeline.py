@@ -134,3 +134,22 @@ def test_pipelined_overflow_and_budget_fallbacks(univ10_store, univ10_oracle):
             tickets.append(dev.batch_submit(pbs[k + 2], v2c=True))
     assert dev.stats()["segments_evicted"] > 0
     dev.close()
+
+
+def test_three_lanes(univ10_store, univ10_oracle):
+    """batch_lanes = 3: three batches in flight, a fourth submit refused."""
+    dev = wg.Device(univ10_store, table_budget=256 << 20, batch_lanes=3)
+    sl = _slices(univ10_store, 6, 80, seed=59)
+    pbs = [wg.PreparedBatch(s) for s in sl]
+    ts = [dev.batch_submit(pbs[k], v2c=True) for k in range(3)]
+    with pytest.raises(wg.WkError) as e:
+        dev.batch_submit(pbs[3], v2c=True)
+    assert e.value.code == wg.WK_E_STATE
+    for k in range(6):
+        dev.batch_wait(ts[k])
+        _check(univ10_oracle, sl[k], dev.batch_fetch(ts[k], pbs[k]))
+        if k + 3 < 6:
+            ts.append(dev.batch_submit(pbs[k + 3], v2c=True))
+    dev.close()
+    with pytest.raises(wg.WkError):
+        wg.Device(univ10_store, batch_lanes=5)

@@ -22,7 +22,8 @@ uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 void free_dev(wk_dev* d) {
   if (!d) return;
   if (d->stream) (void)hipStreamSynchronize(d->stream);
-  if (d->alt.stream) (void)hipStreamSynchronize(d->alt.stream);
+  for (LaneState& l : d->alt)
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
   if (d->copy_stream) (void)hipStreamSynchronize(d->copy_stream);
   for (auto& p : d->pending) {
     (void)hipEventDestroy(p.a);
@@ -30,7 +31,7 @@ void free_dev(wk_dev* d) {
   }
   for (auto e : d->event_pool) (void)hipEventDestroy(e);
   free_lane(*d);
-  free_lane(d->alt);
+  for (LaneState& l : d->alt) free_lane(l);
   void* dev_ptrs[] = {d->karena, d->varena, d->d_segs, d->d_ktab, d->d_vtab};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
@@ -183,6 +184,7 @@ int create_dev(const Store* st, const wk_dev_opts* o, wk_dev* d) {
   if (d->opts.probe_width == 0) d->opts.probe_width = 2;  // fastest measured on the probe at 2,560 universities
   if (d->opts.probe_width != 1 && d->opts.probe_width != 2 && d->opts.probe_width != 4 && d->opts.probe_width != 8)
     return WK_E_INVALID_ARG;
+  if (d->opts.batch_lanes < 0 || d->opts.batch_lanes > kMaxLanes) return WK_E_INVALID_ARG;
   if (d->opts.rows_per_wave != 1 && d->opts.rows_per_wave != 4 && d->opts.rows_per_wave != 16 &&
       d->opts.rows_per_wave != 64)
     d->opts.rows_per_wave = 0;
@@ -325,41 +327,50 @@ int finish_lane(wk_dev* dev) {
   return rc;
 }
 
-// Lane 0 is the device's own, lane 1 the other one (swapped in around the
-// call).
+// Lane 0 is the device's own, lane i > 0 is alt[i - 1] (swapped in around
+// the call).
 template <typename F>
 int on_lane(wk_dev* dev, int lane, F f) {
   if (lane == 0) return f();
-  swap_lanes(dev);
+  swap_lanes(dev, lane);
   const int rc = f();
-  swap_lanes(dev);
+  swap_lanes(dev, lane);
   return rc;
+}
+
+LaneState& lane_state(wk_dev* dev, int lane) {
+  return lane == 0 ? static_cast<LaneState&>(*dev) : dev->alt[lane - 1];
+}
+
+int batch_lanes(const wk_dev* dev) {
+  const int n = dev->opts.batch_lanes;
+  return n <= 0 ? 2 : std::min(n, kMaxLanes);
 }
 
 }  // namespace
 
 namespace wk {
 
-int ensure_alt_lane(wk_dev* dev) {
-  if (dev->alt.stream) return WK_OK;
-  swap_lanes(dev);
+int ensure_alt_lane(wk_dev* dev, int lane) {
+  if (dev->alt[lane - 1].stream) return WK_OK;
+  swap_lanes(dev, lane);
   int rc = WK_OK;
   if (hip_failed(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking))) rc = WK_E_HIP;
   if (!rc) rc = alloc_lane(dev, dev->opts.table_budget);
   if (rc) free_lane(*dev);
-  swap_lanes(dev);
+  swap_lanes(dev, lane);
   return rc;
 }
 
 int drain_batches(wk_dev* dev) {
   int rc = WK_OK;
-  if (dev->inflight) rc = finish_lane(dev);
-  if (dev->alt.inflight) {
-    const int r2 = on_lane(dev, 1, [&]() -> int { return finish_lane(dev); });
-    if (!rc) rc = r2;
+  for (int lane = 0; lane < kMaxLanes; lane++) {
+    if (!lane_state(dev, lane).inflight) continue;
+    const int r = on_lane(dev, lane, [&]() -> int { return finish_lane(dev); });
+    if (!rc) rc = r;
   }
   // the call that drains may reuse the arenas
-  dev->fetchable = dev->alt.fetchable = false;
+  for (int lane = 0; lane < kMaxLanes; lane++) lane_state(dev, lane).fetchable = false;
   return rc;
 }
 
@@ -528,11 +539,13 @@ int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
     return WK_E_INVALID_ARG;
   if (dev->opts.probe_width != probe_width || dev->opts.rows_per_wave != rows_per_wave) {
     (void)hipStreamSynchronize(dev->stream);
-    if (dev->alt.stream) (void)hipStreamSynchronize(dev->alt.stream);
     for (auto& g : dev->graphs) (void)hipGraphExecDestroy(g.second);
     dev->graphs.clear();
-    for (auto& g : dev->alt.graphs) (void)hipGraphExecDestroy(g.second);
-    dev->alt.graphs.clear();
+    for (LaneState& l : dev->alt) {
+      if (l.stream) (void)hipStreamSynchronize(l.stream);
+      for (auto& g : l.graphs) (void)hipGraphExecDestroy(g.second);
+      l.graphs.clear();
+    }
   }
   dev->opts.probe_width = probe_width;
   dev->opts.rows_per_wave = rows_per_wave;
@@ -872,19 +885,27 @@ int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
                     uint32_t* ticket) {
   if (!dev || (!qs && nq) || !valid_result(out) || !ticket) return WK_E_INVALID_ARG;
   *ticket = 0;
-  // the preferred lane if it is free, else the other; both busy: wait first
-  int lane = dev->next_lane;
-  if ((lane == 0 ? dev->inflight : dev->alt.inflight)) lane ^= 1;
-  if ((lane == 0 ? dev->inflight : dev->alt.inflight)) return WK_E_STATE;
-  if (lane == 1) {
-    const int rc = ensure_alt_lane(dev);
+  // the preferred lane if it is free, else the next free one; all busy:
+  // wait first
+  const int nl = batch_lanes(dev);
+  int lane = -1;
+  for (int i = 0; i < nl && lane < 0; i++) {
+    const int c = (dev->next_lane + i) % nl;
+    if (!lane_state(dev, c).inflight) lane = c;
+  }
+  if (lane < 0) return WK_E_STATE;
+  if (lane > 0) {
+    const int rc = ensure_alt_lane(dev, lane);
     if (rc) return rc;
   }
-  if (dev->swapping && (lane == 0 ? dev->alt.inflight : dev->inflight)) {
-    // under a gstore budget smaller than the store, the other lane's pinned
-    // segments could leave this batch no room: finish that one first
-    const int rc = on_lane(dev, lane ^ 1, [&]() -> int { return finish_lane(dev); });
-    if (rc) return rc;
+  if (dev->swapping) {
+    // under a gstore budget smaller than the store, the other lanes' pinned
+    // segments could leave this batch no room: finish theirs first
+    for (int o = 0; o < kMaxLanes; o++) {
+      if (o == lane || !lane_state(dev, o).inflight) continue;
+      const int rc = on_lane(dev, o, [&]() -> int { return finish_lane(dev); });
+      if (rc) return rc;
+    }
   }
   const uint32_t t = dev->next_ticket++;
   if (dev->next_ticket == 0) dev->next_ticket = 1;
@@ -908,15 +929,15 @@ int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
     return WK_OK;
   });
   if (rc) return rc;
-  dev->next_lane = lane ^ 1;
+  dev->next_lane = (lane + 1) % nl;
   *ticket = t;
   return WK_OK;
 }
 
 int wk_batch_wait(wk_dev* dev, uint32_t ticket) {
   if (!dev || ticket == 0) return WK_E_INVALID_ARG;
-  for (int lane = 0; lane < 2; lane++) {
-    LaneState& l = lane == 0 ? static_cast<LaneState&>(*dev) : dev->alt;
+  for (int lane = 0; lane < kMaxLanes; lane++) {
+    LaneState& l = lane_state(dev, lane);
     if (l.ticket != ticket) continue;
     if (!l.inflight) return WK_OK;  // already finished
     return on_lane(dev, lane, [&]() -> int { return finish_lane(dev); });
@@ -926,8 +947,8 @@ int wk_batch_wait(wk_dev* dev, uint32_t ticket) {
 
 int wk_batch_fetch(wk_dev* dev, uint32_t ticket, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {
   if (!dev || ticket == 0) return WK_E_INVALID_ARG;
-  for (int lane = 0; lane < 2; lane++) {
-    LaneState& l = lane == 0 ? static_cast<LaneState&>(*dev) : dev->alt;
+  for (int lane = 0; lane < kMaxLanes; lane++) {
+    LaneState& l = lane_state(dev, lane);
     if (l.ticket != ticket) continue;
     if (l.inflight || !l.fetchable) return WK_E_STATE;  // not waited for, or its arena reused since
     if (used_elems) *used_elems = l.batch_used;

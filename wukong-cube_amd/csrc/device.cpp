@@ -147,7 +147,7 @@ int evict(wk_dev* dev, int v) {
   s.vblk.clear();
   s.resident = false;
   dev->h_segs[v].koff = dev->h_segs[v].voff = kNotResident;
-  dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
+  all_lanes_copy_pending(dev);  // every lane waits for it
   WK_HIP(hipMemcpyAsync(dev->d_segs + v, dev->h_segs + v, sizeof(DevSeg), hipMemcpyHostToDevice,
                         dev->copy_stream));
   dev->stats.segments_evicted++;
@@ -240,7 +240,7 @@ int map_segment(wk_dev* dev, int si) {
   DevSeg& hs = dev->h_segs[si];
   hs.koff = dk / kBucketBytes;
   hs.voff = dv / intptr_t(sizeof(uint32_t));
-  dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
+  all_lanes_copy_pending(dev);  // every lane waits for it
   if (dev->dense[size_t(si)] && !dev->dense_built[size_t(si)]) {
     hs.dvmin = m.vmin;
     hs.dvn = m.vmax - m.vmin + 1;
@@ -282,7 +282,7 @@ int load_segment(wk_dev* dev, int si, const std::vector<int>& keep) {
     if (rc) return rc;
   }
   // blocks being reused may still be read by queued kernels
-  dev->copy_pending = dev->alt.copy_pending = true;  // both lanes wait for it
+  all_lanes_copy_pending(dev);  // every lane waits for it
   WK_HIP(hipEventRecord(dev->compute_done, dev->stream));
   WK_HIP(hipStreamWaitEvent(dev->copy_stream, dev->compute_done, 0));
   const SegMeta& m = dev->st->segs[size_t(si)];

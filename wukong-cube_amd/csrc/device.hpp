@@ -58,6 +58,7 @@ struct SegSlot {
 };
 
 constexpr int kStageBufs = 4;  // pinned staging buffers for swaps from pageable memory
+constexpr int kMaxLanes = 4;   // lanes of a device: its own and up to three others
 
 struct KernelStat {
   uint64_t launches = 0;
@@ -161,9 +162,11 @@ struct wk_dev : wk::LaneState {
   int device = 0;
   hipStream_t copy_stream = nullptr;
   wk_dev_opts opts{};
-  // the other lane: swapped with the device's own fields while its work is
-  // issued (the serving loop's heavy queries)
-  wk::LaneState alt;
+  // the other lanes: one is swapped with the device's own fields while its
+  // work is issued.  alt[0] carries the serving loop's heavy queries; the
+  // pipelined batches rotate over the device's own lane and the first
+  // opts.batch_lanes - 1 of these.
+  wk::LaneState alt[wk::kMaxLanes - 1];
 
   // gstore cache
   uint32_t kshift = 14, vshift = 19;
@@ -233,7 +236,13 @@ namespace wk {
 
 // Exchange the device's own lane fields with dev->alt (the lanes' streams,
 // tables, scratch and descriptors).
-inline void swap_lanes(wk_dev* dev) { std::swap(static_cast<LaneState&>(*dev), dev->alt); }
+inline void swap_lanes(wk_dev* dev, int lane = 1) { std::swap(static_cast<LaneState&>(*dev), dev->alt[lane - 1]); }
+
+// Marks every lane as having copy-stream work to wait for.
+inline void all_lanes_copy_pending(wk_dev* dev) {
+  dev->copy_pending = true;
+  for (LaneState& l : dev->alt) l.copy_pending = true;
+}
 
 // Tables, scratch and step descriptors of the lane now in the device's own
 // fields, for a table budget of `budget` bytes (0: 8 GiB); its stream exists.
@@ -284,9 +293,9 @@ int gather_window(wk_dev* dev, const StepDesc& sd, const uint32_t* tin, uint64_t
 // pool.
 int run_query_to(wk_dev* dev, const Plan& plan, uint32_t* dst, uint64_t cap, uint64_t* rows);
 
-// The other lane's stream, tables and scratch (same table budget), made on
-// first use.
-int ensure_alt_lane(wk_dev* dev);
+// Lane `lane`'s (1 .. kMaxLanes - 1) stream, tables and scratch (same table
+// budget), made on first use.
+int ensure_alt_lane(wk_dev* dev, int lane = 1);
 // Completes every batch submitted with wk_batch_submit and not yet waited
 // for (their result arrays are filled); every entry point that uses a lane
 // calls it first.

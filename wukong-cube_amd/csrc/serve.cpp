@@ -227,7 +227,7 @@ struct HeavyLane {
   }
 
   // Elements of the lane's first input table (the continued tables).
-  uint64_t seed_cap() const { return dev->alt.tcap; }
+  uint64_t seed_cap() const { return dev->alt[0].tcap; }
 
   // Launch the batch `b` on the second lane; `main_tin` is the main lane's
   // current input buffer, which holds the continued tables.  Returns

@@ -72,7 +72,8 @@ class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("gstore_budget", C.c_uint64), ("table_budget", C.c_uint64),
                 ("key_block_shift", C.c_uint32), ("value_block_shift", C.c_uint32),
                 ("enable_pipeline", C.c_int32), ("probe_width", C.c_int32),
-                ("rows_per_wave", C.c_int32), ("profile", C.c_int32), ("dense_index", C.c_int32)]
+                ("rows_per_wave", C.c_int32), ("profile", C.c_int32), ("dense_index", C.c_int32),
+                ("batch_lanes", C.c_int32)]
 
 
 class DevStats(C.Structure):
@@ -530,11 +531,11 @@ class Device:
 
     def __init__(self, store, device=0, gstore_budget=0, table_budget=0, key_block_shift=0,
                  value_block_shift=0, enable_pipeline=1, probe_width=0, rows_per_wave=0, profile=0,
-                 dense_index=0):
+                 dense_index=0, batch_lanes=0):
         self.store = store
         self._h = P()
         o = DevOpts(device, gstore_budget, table_budget, key_block_shift, value_block_shift,
-                    enable_pipeline, probe_width, rows_per_wave, profile, dense_index)
+                    enable_pipeline, probe_width, rows_per_wave, profile, dense_index, batch_lanes)
         _check(lib().wk_dev_create(store.handle, C.byref(o), C.byref(self._h)), "wk_dev_create")
 
     def close(self):
