@@ -325,7 +325,7 @@ int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* 
 /* Pipelined batches (no counterpart in the reference, whose agent runs one
  * query at a time): wk_batch_submit compiles, uploads and launches a batch
  * as wk_batch_run does and returns without waiting for the GPU, so that the
- * caller can prepare and submit the next batch while this one runs.  Two
+ * caller can prepare and submit the next batch while this one runs.  Up to
  * opts.batch_lanes batches (2 by default) can be in flight at once, each on
  * its own lane (stream, history tables, scratch and result arena, allocated
  * with the handle's table budget on a lane's first use); one more submit
@@ -336,8 +336,8 @@ int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* 
  * batches in flight.  wk_batch_fetch copies a waited-for ticket's result
  * arena (offsets as in `out`) to the host until its lane starts another
  * batch or another call uses the lanes (WK_E_STATE after that).  Under a
- * gstore budget smaller than the store, a submit first completes the batch
- * on the other lane, so that its pinned segments cannot starve this one. */
+ * gstore budget smaller than the store, a submit first completes the batches
+ * on the other lanes, so that their pinned segments cannot starve this one. */
 int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph, wk_batch_result* out,
                     uint32_t* ticket);
 int wk_batch_wait(wk_dev* dev, uint32_t ticket);
