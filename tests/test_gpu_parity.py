@@ -512,3 +512,49 @@ def test_direct_key_index_and_buckets_agree(univ10_store, univ10_oracle):
             want, wv, wrc = univ10_oracle.query(pats, nv)
             assert rc == wrc and v2c == wv and same_rows(t, want), (dense, pats)
         dev.close()
+
+
+def _selective_filters(store):
+    """Index starts followed by known_to_const filters that keep about 1/2,
+    1/200 and 1/5,000 of their rows, and a known_to_known join: the gather's
+    output tiles then span 2 to thousands of input rows each."""
+    T = wg.TYPE_ID
+    ug, gs = Q.U("ub:UndergraduateStudent"), Q.U("ub:GraduateStudent")
+    depts = np.sort(store.get_index(Q.U("ub:Department"), wg.IN))
+    courses = np.sort(store.get_index(Q.U("ub:Course"), wg.IN))
+    univs = np.sort(store.get_index(Q.U("ub:University"), wg.IN))
+    qs = []
+    for d in depts[:3]:
+        qs.append(([(ug, T, -1, wg.IN), (-1, Q.U("ub:memberOf"), int(d), wg.OUT)], 1))
+    for c in courses[:3]:
+        qs.append(([(ug, T, -1, wg.IN), (-1, Q.U("ub:takesCourse"), int(c), wg.OUT)], 1))
+    for u in univs[:2]:
+        qs.append(([(gs, T, -1, wg.IN), (-1, Q.U("ub:undergraduateDegreeFrom"), int(u), wg.OUT)], 1))
+    # advisors who are full professors: a k2c by type bitmap
+    qs.append(([(gs, T, -1, wg.IN), (-1, Q.U("ub:advisor"), -2, wg.OUT),
+                (-2, T, Q.U("ub:FullProfessor"), wg.OUT)], 2))
+    # courses taught by the student's advisor: a selective known_to_known join
+    qs.append(([(gs, T, -1, wg.IN), (-1, Q.U("ub:advisor"), -2, wg.OUT), (-1, Q.U("ub:takesCourse"), -3, wg.OUT),
+                (-2, Q.U("ub:teacherOf"), -3, wg.OUT)], 3))
+    return qs
+
+
+def test_selective_filters_alone_and_batched(univ10_dev, univ10_oracle, univ10_store):
+    """Filters of every selectivity, alone and in a batch mixed with the
+    template stream, row for row against the oracle."""
+    qs = _selective_filters(univ10_store)
+    wants = [univ10_oracle.query(p, nv) for p, nv in qs]
+    kept = [w[0].shape[0] for w in wants]
+    assert min(kept) > 0 and max(kept) > 1_000
+    for (pats, nv), (want, wv, wrc) in zip(qs, wants):
+        got, gv, grc = univ10_dev.query(pats, nv)
+        assert grc == wrc == 0 and gv == wv
+        assert same_rows(got, want), pats
+    stream = Q.template_stream(univ10_store, Q.univ_templates(), 200, seed=41)
+    mix = [(p, nv) for p, nv, _ in stream]
+    mix = mix[:100] + qs + mix[100:] + qs[::-1]
+    for rep in range(2):
+        for (pats, nv), (t, v2c, rc) in zip(mix, univ10_dev.batch(mix, use_graph=bool(rep))):
+            want, wv, wrc = univ10_oracle.query(pats, nv)
+            assert rc == wrc == 0 and v2c == wv
+            assert same_rows(t, want), pats

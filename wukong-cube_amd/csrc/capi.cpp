@@ -61,8 +61,8 @@ void free_lane(LaneState& l) {
   for (auto& g : l.graphs) (void)hipGraphExecDestroy(g.second);
   l.graphs.clear();
   // tbuf[1] lies inside tbuf[0]'s allocation (one pool of two tables)
-  void* dev_ptrs[] = {l.tbuf[0], l.arena,  l.cnt,  l.eoff,   l.pfx,  l.tile_sum, l.tile_row,
-                      l.d_desc,  l.d_rs,   l.d_state, l.d_qres, l.d_qagg, l.d_ck,     l.d_win};
+  void* dev_ptrs[] = {l.tbuf[0], l.arena,   l.eoff,   l.pfx,    l.tile_sum, l.tile_row, l.d_desc,
+                      l.d_rs,    l.d_state, l.d_qres, l.d_qagg, l.d_ck,     l.d_win};
   for (void* p : dev_ptrs)
     if (p) (void)hipFree(p);
   void* host_ptrs[] = {l.h_desc, l.h_rs, l.h_state, l.h_qres, l.h_ck, l.h_win};
@@ -74,9 +74,9 @@ void free_lane(LaneState& l) {
 
 int alloc_lane(wk_dev* d, uint64_t budget) {
   // history tables + scratch: 2 ping-pong tables + arena (4 B each per
-  // element) and per-row count/offset/prefix (16 B), rows <= elements
+  // element) and per-row edge offset and prefix (12 B), rows <= elements
   const uint64_t tb = budget ? budget : (uint64_t(8) << 30);
-  d->tcap = std::max<uint64_t>(tb / 28, 1 << 16);
+  d->tcap = std::max<uint64_t>(tb / 24, 1 << 16);
   d->arena_cap = d->tcap;
   d->rcap = d->tcap;
   d->tile_cap = d->rcap / kTile + 2;
@@ -88,10 +88,9 @@ int alloc_lane(wk_dev* d, uint64_t budget) {
   WK_HIP(hipMalloc(&d->d_win, size_t(kWinGroup) * 3 * sizeof(uint64_t)));
   WK_HIP(hipHostMalloc(&d->h_win, size_t(kWinGroup) * 3 * sizeof(uint64_t), hipHostMallocDefault));
   WK_HIP(hipMalloc(&d->arena, d->arena_cap * sizeof(uint32_t)));
-  WK_HIP(hipMalloc(&d->cnt, d->rcap * sizeof(uint32_t)));
   WK_HIP(hipMalloc(&d->eoff, d->rcap * sizeof(uint32_t)));
   WK_HIP(hipMalloc(&d->pfx, (d->rcap + 1) * sizeof(uint64_t)));
-  WK_HIP(hipMalloc(&d->tile_sum, d->tile_cap * sizeof(uint64_t)));
+  WK_HIP(hipMalloc(&d->tile_sum, (d->tile_cap + 1) * sizeof(uint64_t)));
   WK_HIP(hipMalloc(&d->tile_row, (d->otile_cap + 1) * sizeof(uint64_t)));
   return reserve_steps(d, 8, 64);
 }

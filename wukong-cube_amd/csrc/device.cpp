@@ -448,9 +448,8 @@ StepArgs make_args(wk_dev* dev, uint32_t step, uint32_t nq, const uint32_t* tin,
   a.row_start = dev->d_rs + size_t(step) * (dev->nq_cap + 1);
   a.next_row_start = has_next ? dev->d_rs + size_t(step + 1) * (dev->nq_cap + 1) : nullptr;
   a.nq = nq;
-  a.cnt = dev->cnt;
   a.eoff = dev->eoff;
-  a.pfx = dev->pfx;
+  a.lpfx = dev->pfx;
   a.rcap = dev->rcap;
   a.tile_sum = dev->tile_sum;
   a.tile_cap = dev->tile_cap;
@@ -567,8 +566,8 @@ int probe_scan(wk_dev* dev, const StepDesc& sd0, const uint32_t* tin, uint64_t R
   const double Rd = double(R);
   hipEvent_t ev;
   // per-kernel HIP-event timing (opts.profile) with the algorithmic bytes of
-  // each launch: key column 4 B + slot 16 B + cnt/eoff 8 B per row for the
-  // probe, cnt 4 B + prefix 8 B per row for the row scan
+  // each launch: key column 4 B + slot 16 B + prefix 8 B and edge offset 4 B
+  // per row for the probe
   const double ntiles = std::ceil(Rd / double(kTile));
   // a key read from a direct index is one 8-B word, so lanes sharing a key
   // would only repeat it: one lane per key
@@ -577,13 +576,10 @@ int probe_scan(wk_dev* dev, const StepDesc& sd0, const uint32_t* tin, uint64_t R
   launch_probe(a, direct ? 1 : dev->opts.probe_width, R, dev->stream);
   // direct key index: 8-B ptr word instead of the 16-B slot; a type bitmap:
   // one 4-B word
-  timed_end(dev, ev, "k_probe", (sd.op == OP_FILTER_TYPE ? 16 : direct ? 20 : 28) * Rd);
+  timed_end(dev, ev, "k_probe", (sd.op == OP_FILTER_TYPE ? 20 : direct ? 24 : 32) * Rd);
   timed_begin(dev, &ev);
   launch_scan_tiles(a, dev->stream);
   timed_end(dev, ev, "k_scan_tiles", 16 * ntiles);
-  timed_begin(dev, &ev);
-  launch_scan_rows(a, R, dev->stream);
-  timed_end(dev, ev, "k_scan_rows", 12 * Rd);
   timed_begin(dev, &ev);
   launch_scan_queries(a, dev->stream);
   timed_end(dev, ev, "k_scan_queries", 96);
@@ -722,7 +718,7 @@ struct QueryExec {
         fresh = true;
       }
       // each chunk's first and last input row, and the prefix at the first
-      launch_window_rows(dev->pfx, R, g0 * C, C, T, n, dev->d_win, dev->stream);
+      launch_window_rows(dev->tile_sum, dev->pfx, R, g0 * C, C, T, n, dev->d_win, dev->stream);
       if (hip_failed(hipMemcpyAsync(dev->h_win, dev->d_win, size_t(n) * 3 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                     dev->stream)) ||
           hip_failed(hipStreamSynchronize(dev->stream))) {

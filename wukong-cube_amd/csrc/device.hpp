@@ -107,9 +107,8 @@ struct LaneState {
   uint64_t tcap = 0;
   uint32_t* arena = nullptr;
   uint64_t arena_cap = 0;
-  uint32_t* cnt = nullptr;
   uint32_t* eoff = nullptr;
-  uint64_t* pfx = nullptr;
+  uint64_t* pfx = nullptr;  // a step's prefixes within tiles (StepArgs.lpfx); scratch of the split
   uint64_t rcap = 0;
   uint64_t* tile_sum = nullptr;
   uint64_t tile_cap = 0;

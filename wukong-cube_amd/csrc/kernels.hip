@@ -86,6 +86,13 @@ __device__ __forceinline__ uint32_t edge_at(const StepArgs& a, const DevSeg& s, 
   return a.varena[i];
 }
 
+// Output rows of the step before input row g (g <= R), once k_scan_tiles has
+// run: its tile's prefix plus its prefix within the tile.
+__device__ __forceinline__ uint64_t pfx_at(const StepArgs& a, uint64_t g) {
+  return a.tile_sum[g >> kTileShift] + a.lpfx[g];
+}
+
+
 // Largest q in [ql, qh) with rs[q] <= g, given rs[ql] <= g.
 __device__ __forceinline__ uint32_t query_in(const uint64_t* rs, uint32_t ql, uint32_t qh, uint64_t g) {
   while (qh - ql > 1) {
@@ -143,11 +150,12 @@ __device__ __forceinline__ void block_find_queries(const uint64_t* rs, uint32_t 
   q1 = lo[1];
 }
 
-// Largest g in [0, n) with v[g] <= x, for two values x0 <= x1 at once, by
-// all NT threads of a block (v ascending, v[0] <= x0): block_find_queries
-// over a 64-bit index range.  Every thread of the block must call it.
+// Largest g in [0, n) with pfx_at(a, g) <= x, for two values x0 <= x1 at
+// once, by all NT threads of a block (pfx_at(a, 0) = 0 <= x0):
+// block_find_queries over a 64-bit index range.  Every thread of the block
+// must call it.
 template <int NT>
-__device__ __forceinline__ void block_find_rows(const uint64_t* v, uint64_t n, uint64_t x0, uint64_t x1,
+__device__ __forceinline__ void block_find_rows(const StepArgs& a, uint64_t n, uint64_t x0, uint64_t x1,
                                                 uint64_t& g0, uint64_t& g1) {
   __shared__ uint32_t s_at[2][2];
   uint64_t lo[2] = {0, 0}, len[2] = {n, n};
@@ -158,7 +166,8 @@ __device__ __forceinline__ void block_find_rows(const uint64_t* v, uint64_t n, u
     for (int k = 0; k < 2; ++k) {
       st[k] = (len[k] + NT - 1) / NT;
       const uint64_t o = uint64_t(threadIdx.x) * st[k];
-      if (len[k] > 1 && o < len[k] && v[lo[k] + o] <= x[k] && (o + st[k] >= len[k] || v[lo[k] + o + st[k]] > x[k]))
+      if (len[k] > 1 && o < len[k] && pfx_at(a, lo[k] + o) <= x[k] &&
+          (o + st[k] >= len[k] || pfx_at(a, lo[k] + o + st[k]) > x[k]))
         s_at[round & 1][k] = threadIdx.x;
     }
     __syncthreads();
@@ -437,12 +446,12 @@ __device__ __forceinline__ void probe_row(const StepArgs& a, const ProbeDesc& d,
 }
 
 // Exclusive scan of the per-tile edge counts by one block of NT threads;
-// sets the step's total output rows.
+// sets the step's total output rows.  Entry nt of the tile prefixes and
+// lpfx[R] are set so that pfx_at(a, R) is the total too.
 template <int NT>
 __device__ __forceinline__ void scan_tiles_block(const StepArgs& a) {
   const uint64_t R = step_rows(a);
-  uint64_t nt = (R + kTile - 1) / kTile;
-  if (nt > a.tile_cap) nt = a.tile_cap;
+  const uint64_t nt = (R + kTile - 1) / kTile;  // R <= rcap, so nt < tile_cap
   const uint64_t per = (nt + NT - 1) / NT;
   uint64_t lo = uint64_t(threadIdx.x) * per;
   if (lo > nt) lo = nt;
@@ -454,9 +463,15 @@ __device__ __forceinline__ void scan_tiles_block(const StepArgs& a) {
   for (uint64_t i = lo; i < hi; ++i) {
     const uint64_t v = a.tile_sum[i];
     a.tile_sum[i] = ex;
+    // row R inside the last tile: the tile's rows before it are all of them
+    if (i == nt - 1 && (R & (kTile - 1))) a.lpfx[R] = v;
     ex += v;
   }
-  if (threadIdx.x == 0) a.st->total_out_rows = total;
+  if (threadIdx.x == 0) {
+    a.st->total_out_rows = total;
+    a.tile_sum[nt] = total;
+    if (!(R & (kTile - 1))) a.lpfx[R] = 0;  // row R starts tile nt
+  }
 }
 
 // Two rows of one query at once: both keys, then both first buckets are
@@ -525,17 +540,24 @@ template <int W, int NT>
 __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
   static_assert(kTile % NT == 0 && NT % W == 0, "probe tile shape");
   if (a.st->status) return;
-  __shared__ uint64_t s_rs[NT];
-  __shared__ ProbeDesc s_pd[NT];
-  __shared__ uint64_t s_in[NT];
-  __shared__ DevSeg s_sg[NT];
+  // the queries of a tile that spans several, staged by the batch probe
+  // only (a one-query launch has no such tile)
+  constexpr int QS = NT == 1024 ? NT : 1;
+  __shared__ uint64_t s_rs[QS];
+  __shared__ ProbeDesc s_pd[QS];
+  __shared__ uint64_t s_in[QS];
+  __shared__ DevSeg s_sg[QS];
+  // the tile's counts, scanned at its end; with several rows per thread the
+  // prefixes go through s_pfx, so that their stores are coalesced
+  constexpr int kPer = kTile / NT;
+  __shared__ uint32_t s_cnt[kTile];
+  __shared__ uint64_t s_pfx[kPer > 1 ? kTile : 1];
   const uint64_t R = step_rows(a);
   const uint64_t ntiles = (R + kTile - 1) / kTile;
   constexpr int G = NT / W;
   constexpr int kPasses = kTile / G;
   const int grp = threadIdx.x / W, gl = threadIdx.x % W;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    uint64_t tsum = 0;
     const uint64_t tlo = t * kTile, thi = (tlo + kTile < R) ? tlo + kTile : R;
     uint32_t q0, q1;
     block_find_queries<NT>(a.row_start, a.nq, tlo, thi - 1, q0, q1);
@@ -566,13 +588,11 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
               probe_row2<W>(a, d, s, tbase + (g - r0) * d.in_cols, tbase + (gr1 - r0) * d.in_cols, c, eo, c1, eo1);
             }
             if (gl == 0) {
-              a.cnt[g] = c;
+              s_cnt[g - tlo] = c;
               a.eoff[g] = eo;
-              tsum += c;
               if (has1) {
-                a.cnt[g1] = c1;
+                s_cnt[g1 - tlo] = c1;
                 a.eoff[g1] = eo1;
-                tsum += c1;
               }
             }
           }
@@ -585,16 +605,15 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
             if (tb) c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
             else if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
             if (gl == 0) {
-              a.cnt[g] = c;
+              s_cnt[g - tlo] = c;
               a.eoff[g] = eo;
-              tsum += c;
             }
           }
         }
       }
     } else {
       const uint32_t nqt = q1 - q0 + 1;
-      const bool staged = nqt <= uint32_t(NT);
+      const bool staged = QS > 1 && nqt <= uint32_t(QS);
       if (staged && threadIdx.x < nqt) {
         // the tile's queries: row start, probe descriptor, input offset and
         // segment, so that a row's chain starts from LDS instead of three
@@ -634,15 +653,47 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
                            eo);
           }
           if (gl == 0) {
-            a.cnt[g] = c;
+            s_cnt[g - tlo] = c;
             a.eoff[g] = eo;
-            tsum += c;
           }
         }
       }
     }
-    tsum = block_sum<NT>(tsum);
-    if (threadIdx.x == 0 && t < a.tile_cap) a.tile_sum[t] = tsum;
+    // the tile's prefix of counts, kPer consecutive rows per thread, and its
+    // sum
+    __syncthreads();
+    const uint32_t nrow = uint32_t(thi - tlo);
+    // this thread's index, opaque to the compiler, so that the addresses below
+    // are formed here and not kept in registers across the whole tile loop
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    uint32_t cc[kPer];  // a row's count is below 2^28
+    uint64_t cs = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t i = tid * kPer + k;
+      cc[k] = i < nrow ? s_cnt[i] : 0u;
+      cs += cc[k];
+    }
+    uint64_t tsum = 0;
+    uint64_t ex = block_excl_scan<NT>(cs, &tsum);  // ends with a barrier: s_cnt is free again
+    if constexpr (kPer == 1) {
+      if (tid < nrow) a.lpfx[tlo + tid] = ex;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        s_pfx[tid * kPer + k] = ex;
+        ex += cc[k];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = k * NT + tid;
+        if (i < nrow) a.lpfx[tlo + i] = s_pfx[i];
+      }
+      __syncthreads();  // before the next tile's prefixes overwrite s_pfx
+    }
+    if (threadIdx.x == 0) a.tile_sum[t] = tsum;
   }
 }
 
@@ -651,33 +702,6 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
 __global__ __launch_bounds__(1024) void k_scan_tiles(StepArgs a) {
   if (a.st->status) return;
   scan_tiles_block<1024>(a);
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_rows(StepArgs a) {
-  if (a.st->status) return;
-  const uint64_t R = step_rows(a);
-  uint64_t nt = (R + kTile - 1) / kTile;
-  if (nt > a.tile_cap) nt = a.tile_cap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.pfx[R] = a.st->total_out_rows;
-  for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
-    const uint64_t g0 = t * kTile + uint64_t(threadIdx.x) * kRowsPerThread;
-    uint32_t c[kRowsPerThread];
-    uint64_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kRowsPerThread; ++j) {
-      c[j] = (g0 + j < R) ? a.cnt[g0 + j] : 0u;
-      s += c[j];
-    }
-    uint64_t ex = block_excl_scan<kBlock>(s, nullptr) + a.tile_sum[t];
-#pragma unroll
-    for (int j = 0; j < kRowsPerThread; ++j) {
-      const uint64_t g = g0 + j;
-      if (g < R) {
-        a.pfx[g] = ex;
-        ex += c[j];
-      }
-    }
-  }
 }
 
 // End of the output rows the gather writes in this launch (the step's total,
@@ -714,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
     uint64_t lo = 0, hi = R;
     while (hi - lo > 1) {
       const uint64_t mid = (lo + hi) >> 1;
-      if (a.pfx[mid] <= j) lo = mid;
+      if (pfx_at(a, mid) <= j) lo = mid;
       else hi = mid;
     }
     a.tile_row[t] = lo;
@@ -751,7 +775,7 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
     uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
     r0 = r0 < R ? r0 : R;
     r1 = r1 < R ? r1 : R;
-    return a.pfx[r1] - a.pfx[r0];
+    return pfx_at(a, r1) - pfx_at(a, r0);
   };
   // up to kPer queries per thread (nq <= 8192): their row counts are loaded
   // together and kept in registers for the second pass
@@ -773,7 +797,7 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
     }
     uint64_t pf[kPer + 1];
 #pragma unroll
-    for (uint32_t i = 0; i <= kPer; ++i) pf[i] = a.pfx[rs[i]];
+    for (uint32_t i = 0; i <= kPer; ++i) pf[i] = pfx_at(a, rs[i]);
     uint32_t dc[kPer], df[kPer];
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
@@ -897,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void k_query_sums(StepArgs a, uint32_t nqb)
       uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
       r0 = r0 < R ? r0 : R;
       r1 = r1 < R ? r1 : R;
-      const uint64_t rows = a.pfx[r1] - a.pfx[r0];
+      const uint64_t rows = pfx_at(a, r1) - pfx_at(a, r0);
       const uint32_t oc = a.desc[q].out_cols;
       if (a.desc[q].final_step) {
         ea = rows * oc;
@@ -924,7 +948,7 @@ __global__ __launch_bounds__(kBlock) void k_query_sums(StepArgs a, uint32_t nqb)
     uint64_t lo = 0, hi = R;
     while (hi - lo > 1) {
       const uint64_t mid = (lo + hi) >> 1;
-      if (a.pfx[mid] <= j) lo = mid;
+      if (pfx_at(a, mid) <= j) lo = mid;
       else hi = mid;
     }
     a.tile_row[t] = lo;
@@ -976,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_queries_mb(StepArgs a, uint32_t
     uint64_t r0 = a.row_start[q], r1 = a.row_start[q + 1];
     r0 = r0 < R ? r0 : R;
     r1 = r1 < R ? r1 : R;
-    rows = a.pfx[r1] - a.pfx[r0];
+    rows = pfx_at(a, r1) - pfx_at(a, r0);
     oc = a.desc[q].out_cols;
     fin = a.desc[q].final_step != 0;
   }
@@ -1061,12 +1085,12 @@ template <bool CK>
 __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j, uint64_t pg,
                                              uint32_t eo) {
   const uint64_t r0 = a.row_start[q];
-  return emit_row<CK>(a, q, r0, a.pfx[r0], g, j, pg, eo);
+  return emit_row<CK>(a, q, r0, pfx_at(a, r0), g, j, pg, eo);
 }
 
 template <bool CK>
 __device__ __forceinline__ uint64_t emit_row(const StepArgs& a, uint32_t q, uint64_t g, uint64_t j) {
-  return emit_row<CK>(a, q, g, j, a.pfx[g], a.eoff[g]);
+  return emit_row<CK>(a, q, g, j, pfx_at(a, g), a.eoff[g]);
 }
 
 __device__ __forceinline__ void add_checksum(const StepArgs& a, uint32_t q, uint64_t h) {
@@ -1075,20 +1099,97 @@ __device__ __forceinline__ void add_checksum(const StepArgs& a, uint32_t q, uint
 
 // Output-balanced: a block writes one tile of kOutTile consecutive output
 // rows, kOutTile / kBlock per thread in coalesced order.  The tile's source
-// rows [lo, hi) come from the tile -> row map; when they fit, their prefix and
-// edge offsets are staged in LDS and each output row finds its source row by
-// a binary search there (a tile of a heavy row stages one entry).  Tiles of
-// many empty rows (selective filters) search the global prefix instead.
+// rows [lo, hi) come from the tile -> row map.  When they fit, their prefix
+// and edge offsets are staged in LDS and each output row finds its source row
+// by a binary search there (a tile of a heavy row stages one entry).  When
+// they do not (filters that keep a fraction of their rows), every S-th row's
+// prefix is staged, S = ceil(rows / kOutTile): the search in LDS brackets the
+// source row within S rows, and log2(S) loads of the global prefix finish it.
 constexpr int kStage = kOutTile + 1;
 constexpr int kRegCols = 8;  // input columns a gather thread holds in registers
 constexpr int kQStage = kBlock;  // queries of a mixed tile staged in LDS
 
+// Last row g in [g0, g1) whose prefix is <= j, given the prefix p of g0 is;
+// returns g and sets p to its prefix.
+__device__ __forceinline__ uint64_t refine_row(const StepArgs& a, uint64_t g0, uint64_t g1, uint64_t j,
+                                               uint64_t& p) {
+  while (g1 - g0 > 1) {
+    const uint64_t m = (g0 + g1) >> 1;
+    const uint64_t v = pfx_at(a, m);
+    if (v <= j) {
+      g0 = m;
+      p = v;
+    } else {
+      g1 = m;
+    }
+  }
+  return g0;
+}
+
+// The rows of a tile that spans several queries: each output row finds its
+// source row (in the staged rows, or bracketed by the staged samples and
+// finished in the global prefix) and its query (in LDS when the tile's
+// queries are staged there, else in the row starts).  One instantiation per
+// kind of staging, so that every source of a row's values is read in its own
+// code and only values meet.
+template <bool CK, bool STAGED>
+__device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uint64_t total, uint64_t lo,
+                                             uint64_t hi, uint64_t S, uint32_t ns, uint32_t qlo, uint32_t qhi,
+                                             bool qlds, const int32_t* s_rel, const uint32_t* s_eoff,
+                                             const uint64_t* s_qr0, const uint64_t* s_qp0,
+                                             unsigned long long* s_ck) {
+#pragma unroll
+  for (int k = 0; k < kOutTile / kBlock; ++k) {
+    const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+    if (j >= total) continue;
+    const int32_t jl = int32_t(j - j0);
+    uint32_t l = 0, h = ns;
+    while (h - l > 1) {
+      const uint32_t m = (l + h) >> 1;
+      if (s_rel[m] <= jl) l = m;
+      else h = m;
+    }
+    // the source row g, its first output row pg and its edge offset eo
+    uint64_t pg = uint64_t(int64_t(j0) + s_rel[l]);
+    uint64_t g;
+    uint32_t eo;
+    if constexpr (STAGED) {
+      g = lo + l;
+      eo = s_eoff[l];
+    } else {
+      const uint64_t gl = lo + uint64_t(l) * S;
+      g = refine_row(a, gl, gl + S < hi ? gl + S : hi, j, pg);
+      eo = a.eoff[g];
+    }
+    if (qlds) {
+      uint32_t ql = 0, qh = qhi - qlo;
+      while (qh - ql > 1) {
+        const uint32_t m = (ql + qh) >> 1;
+        if (s_qr0[m] <= g) ql = m;
+        else qh = m;
+      }
+      // checksum shares gather in LDS, one global add per query and tile
+      const uint64_t rh = emit_row<CK>(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, pg, eo);
+      if (CK && rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
+    } else {
+      const uint32_t q = query_in(a.row_start, qlo, qhi, g);
+      const uint64_t rh = emit_row<CK>(a, q, g, j, pg, eo);
+      if (CK) add_checksum(a, q, rh);
+    }
+  }
+  if (CK && a.ck && qlds) {
+    __syncthreads();
+    if (threadIdx.x < qhi - qlo && s_ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], s_ck[threadIdx.x]);
+  }
+}
+
 template <bool CK>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   if (a.st->status) return;
-  // the tile's source rows: their first output row relative to the tile's
-  // first, pfx - j0, in (-2^28, kOutTile] for every row holding one of the
-  // tile's output rows (a row has fewer than 2^28 edges), clamped above
+  // the tile's source rows (or every S-th of them): their first output row
+  // relative to the tile's first, pfx - j0, in (-2^28, kOutTile] for every
+  // row holding one of the tile's output rows (a row has fewer than 2^28
+  // edges), clamped above
   __shared__ int32_t s_rel[kStage];
   __shared__ uint32_t s_eoff[kStage];
   __shared__ uint64_t s_qr0[kQStage];  // first input row of each query of the tile
@@ -1099,6 +1200,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   const uint64_t nt = out_tiles(a);
   for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
     const uint64_t j0 = a.win_lo + t * kOutTile;
+    // this thread's index, opaque to the compiler, so that the values derived
+    // from it are formed per tile and not held in registers across the loop
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     uint64_t lo, hi;
     if (t + 1 < a.otile_cap) {
       lo = a.tile_row[t];
@@ -1107,19 +1212,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       // past the tile map (dropped final tables before this tile): the rows
       // holding this tile's first row and the next tile's
       uint64_t r1;
-      block_find_rows<kBlock>(a.pfx, R, j0, j0 + kOutTile, lo, r1);
+      block_find_rows<kBlock>(a, R, j0, j0 + kOutTile, lo, r1);
       hi = (t + 1 < nt) ? r1 + 1 : R;
     }
     const uint64_t nr = hi - lo;
     const bool staged = nr <= uint64_t(kStage);
+    // staged: every row (S = 1); else every S-th row, ns <= kOutTile samples
+    const uint64_t S = staged ? 1 : (nr + kOutTile - 1) / kOutTile;
+    const uint32_t ns = uint32_t(staged ? nr : (nr + S - 1) / S);
+    // unconditional loads at clamped indices (rows lo..hi-1 exist), so all
+    // of them are in flight together
     if (staged) {
-      // unconditional loads at clamped indices (rows lo..hi-1 exist), so all
-      // of them are in flight together
 #pragma unroll
       for (int k = 0; k < (kStage + kBlock - 1) / kBlock; ++k) {
-        const uint32_t i = threadIdx.x + k * kBlock;
+        const uint32_t i = tid + k * kBlock;
         const uint64_t gi = lo + (i < uint32_t(nr) ? i : uint32_t(nr) - 1);
-        const uint64_t pv = a.pfx[gi];
+        const uint64_t pv = pfx_at(a, gi);
         const uint32_t ev = a.eoff[gi];
         if (i < uint32_t(nr)) {
           const int64_t r = int64_t(pv - j0);
@@ -1127,17 +1235,29 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
           s_eoff[i] = ev;
         }
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < (kOutTile + kBlock - 1) / kBlock; ++k) {
+        const uint32_t i = tid + k * kBlock;
+        const uint64_t pv = pfx_at(a, lo + uint64_t(i < ns ? i : ns - 1) * S);
+        if (i < ns) {
+          const int64_t r = int64_t(pv - j0);
+          s_rel[i] = int32_t(r < (int64_t(1) << 30) ? r : (int64_t(1) << 30));
+        }
+      }
     }
     // the queries of rows lo and hi-1 bound every row's query in this tile
     uint32_t qlo, qlast;
     block_find_queries<kBlock>(a.row_start, a.nq, lo, hi - 1, qlo, qlast);
     const uint32_t qhi = qlast + 1;
-    const bool qstaged = qhi - qlo <= uint32_t(kQStage);
-    if (qhi - qlo > 1 && qstaged && threadIdx.x < qhi - qlo) {
-      const uint64_t r0 = a.row_start[qlo + threadIdx.x];
-      s_qr0[threadIdx.x] = r0;
-      s_qp0[threadIdx.x] = a.pfx[r0 < R ? r0 : R];
-      s_ck[threadIdx.x] = 0;
+    // the queries of a tile that spans several: first row and first output
+    // row of each, staged in LDS when they fit
+    const bool qlds = qhi - qlo > 1 && qhi - qlo <= uint32_t(kQStage);
+    if (qlds && tid < qhi - qlo) {
+      const uint64_t r0 = a.row_start[qlo + tid];
+      s_qr0[tid] = r0;
+      s_qp0[tid] = pfx_at(a, r0 < R ? r0 : R);
+      s_ck[tid] = 0;
     }
     __syncthreads();
     if (staged && qhi - qlo == 1 && a.desc[qlo].out_off == kNoSpace) {
@@ -1148,7 +1268,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       const uint32_t q = __builtin_amdgcn_readfirstlane(qlo);
       const StepDesc& d = a.desc[q];
       const uint64_t r0 = a.row_start[q];
-      const uint64_t p0 = a.pfx[r0];
+      const uint64_t p0 = pfx_at(a, r0);
       const uint32_t ic = d.in_cols, oc = d.out_cols;
       const bool expand = d.op == OP_EXPAND;
       uint32_t* const dbase = (d.final_step ? a.arena : a.tout) + d.out_off;
@@ -1162,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       bool act[kR];
 #pragma unroll
       for (int k = 0; k < kR; ++k) {
-        const int32_t jl = k * kBlock + threadIdx.x;
+        const int32_t jl = k * kBlock + tid;
         act[k] = j0 + uint64_t(jl) < total;
         uint32_t l = 0, h = uint32_t(nr);
         while (h - l > 1) {
@@ -1183,14 +1303,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
           const uint32_t* src = sbase + (lo + ls[k] - r0) * ic;  // an input row of the tile
 #pragma unroll
           for (int c = 0; c < kRegCols; ++c) v[k][c] = uint32_t(c) < ic ? src[c] : 0u;
-          const int32_t jl = k * kBlock + threadIdx.x;
+          const int32_t jl = k * kBlock + tid;
           e[k] = (expand && act[k]) ? edge_at(a, sg, uint64_t(s_eoff[ls[k]]) + uint64_t(int64_t(jl) - s_rel[ls[k]]))
                                     : 0u;
         }
 #pragma unroll
         for (int k = 0; k < kR; ++k) {
           if (!act[k]) continue;
-          const uint32_t jl = k * kBlock + threadIdx.x;
+          const uint32_t jl = k * kBlock + tid;
           uint32_t* dst = dbase + (j0 + jl - p0 - a.win_lo) * oc;
           uint64_t rh = row_hash_init(oc);
 #pragma unroll
@@ -1210,7 +1330,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < kR; ++k) {
           if (!act[k]) continue;
-          const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+          const uint64_t j = j0 + uint64_t(k) * kBlock + tid;
           const uint32_t l = ls[k];
           const uint64_t g = lo + l;
           uint32_t* dst = dbase + (j - p0 - a.win_lo) * oc;
@@ -1231,62 +1351,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       }
       if (ck) {
         hsum = block_sum<kBlock>(hsum);
-        if (threadIdx.x == 0 && hsum) atomicAdd(&a.ck[q], (unsigned long long)hsum);
+        if (tid == 0 && hsum) atomicAdd(&a.ck[q], (unsigned long long)hsum);
       }
+    } else if (staged) {
+      gather_mixed<CK, true>(a, j0, total, lo, hi, 1, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qr0, s_qp0, s_ck);
     } else {
-#pragma unroll
-      for (int k = 0; k < kOutTile / kBlock; ++k) {
-        const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
-        if (j >= total) continue;
-        // each branch writes its row itself, so no pointer into LDS or global
-        // memory is carried out of it
-        if (staged && qstaged) {
-          const int32_t jl = int32_t(j - j0);
-          uint32_t l = 0, h = uint32_t(nr);
-          while (h - l > 1) {
-            const uint32_t m = (l + h) >> 1;
-            if (s_rel[m] <= jl) l = m;
-            else h = m;
-          }
-          const uint64_t g = lo + l;
-          uint32_t ql = 0, qh = qhi - qlo;
-          while (qh - ql > 1) {
-            const uint32_t m = (ql + qh) >> 1;
-            if (s_qr0[m] <= g) ql = m;
-            else qh = m;
-          }
-          // checksum shares gather in LDS, one global add per query and tile
-          const uint64_t rh =
-              emit_row<CK>(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, uint64_t(int64_t(j0) + s_rel[l]), s_eoff[l]);
-          if (CK && rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
-        } else if (staged) {
-          const int32_t jl = int32_t(j - j0);
-          uint32_t l = 0, h = uint32_t(nr);
-          while (h - l > 1) {
-            const uint32_t m = (l + h) >> 1;
-            if (s_rel[m] <= jl) l = m;
-            else h = m;
-          }
-          const uint64_t g = lo + l;
-          const uint32_t q = query_in(a.row_start, qlo, qhi, g);
-          const uint64_t rh = emit_row<CK>(a, q, g, j, uint64_t(int64_t(j0) + s_rel[l]), s_eoff[l]);
-          if (CK) add_checksum(a, q, rh);
-        } else {
-          uint64_t l = lo, h = hi;
-          while (h - l > 1) {
-            const uint64_t m = (l + h) >> 1;
-            if (a.pfx[m] <= j) l = m;
-            else h = m;
-          }
-          const uint32_t q = query_in(a.row_start, qlo, qhi, l);
-          const uint64_t rh = emit_row<CK>(a, q, l, j, a.pfx[l], a.eoff[l]);
-          if (CK) add_checksum(a, q, rh);
-        }
-      }
-      if (CK && a.ck && qhi - qlo > 1 && qstaged) {
-        __syncthreads();
-        if (threadIdx.x < qhi - qlo && s_ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], s_ck[threadIdx.x]);
-      }
+      gather_mixed<CK, false>(a, j0, total, lo, hi, S, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qr0, s_qp0, s_ck);
     }
     __syncthreads();
   }
@@ -1300,10 +1370,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_rows(StepArgs a) {
   constexpr int G = kBlock / LPR;
   const int grp = threadIdx.x / LPR, gl = threadIdx.x % LPR;
   for (uint64_t g = uint64_t(blockIdx.x) * G + grp; g < R; g += uint64_t(gridDim.x) * G) {
-    const uint32_t c = a.cnt[g];
+    const uint64_t p = pfx_at(a, g);
+    const uint64_t c = pfx_at(a, g + 1) - p;
     if (!c) continue;
     const uint32_t q = find_query(a.row_start, a.nq, g);
-    const uint64_t p = a.pfx[g];
     uint64_t hsum = 0;
     const uint64_t e = out_end(a);
     for (uint32_t k = gl; k < c; k += LPR)
@@ -1570,27 +1640,34 @@ __global__ __launch_bounds__(kBlock) void k_dense_build(const Slot* karena, cons
   }
 }
 
-// Largest g < R with pfx[g] <= j (pfx[0] = 0 <= j).
-__device__ __forceinline__ uint64_t row_of_output(const uint64_t* pfx, uint64_t R, uint64_t j) {
+// Prefix of row g from the tile prefixes and the prefixes within tiles.
+__device__ __forceinline__ uint64_t prefix_of(const uint64_t* tile_sum, const uint64_t* lpfx, uint64_t g) {
+  return tile_sum[g >> kTileShift] + lpfx[g];
+}
+
+// Largest g < R whose prefix is <= j (row 0's is 0 <= j).
+__device__ __forceinline__ uint64_t row_of_output(const uint64_t* tile_sum, const uint64_t* lpfx, uint64_t R,
+                                                  uint64_t j) {
   uint64_t lo = 0, hi = R;
   while (hi - lo > 1) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (pfx[mid] <= j) lo = mid;
+    if (prefix_of(tile_sum, lpfx, mid) <= j) lo = mid;
     else hi = mid;
   }
   return lo;
 }
 
-__global__ __launch_bounds__(kBlock) void k_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C,
-                                                        uint64_t T, uint32_t n, uint64_t* out) {
+__global__ __launch_bounds__(kBlock) void k_window_rows(const uint64_t* tile_sum, const uint64_t* lpfx, uint64_t R,
+                                                        uint64_t j0, uint64_t C, uint64_t T, uint32_t n,
+                                                        uint64_t* out) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const uint64_t j = j0 + uint64_t(i) * C;
   const uint64_t je = (j + C < T ? j + C : T) - 1;
-  const uint64_t ra = row_of_output(pfx, R, j);
+  const uint64_t ra = row_of_output(tile_sum, lpfx, R, j);
   out[3 * uint64_t(i)] = ra;
-  out[3 * uint64_t(i) + 1] = row_of_output(pfx, R, je);
-  out[3 * uint64_t(i) + 2] = pfx[ra];
+  out[3 * uint64_t(i) + 1] = row_of_output(tile_sum, lpfx, R, je);
+  out[3 * uint64_t(i) + 2] = prefix_of(tile_sum, lpfx, ra);
 }
 
 inline unsigned grid_for(uint64_t units, unsigned cap) {
@@ -1605,14 +1682,12 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
 // workgroup holds 120 KB of LDS, one per CU, so 2,048 of them took 8 rounds.
 struct LaunchGrids {
   unsigned probe[4];       // k_probe<W, 1024>, W = 1, 2, 4, 8
-  unsigned scan_rows;
   unsigned tile_rows;
   unsigned gather[2];      // k_gather<false>, k_gather<true>
   unsigned gather_rows[4]; // k_gather_rows<1, 4, 16, 64>
   unsigned query_sums;
 };
 LaunchGrids g_grids = {{kPersistentGrid, kPersistentGrid, kPersistentGrid, kPersistentGrid},
-                       kPersistentGrid,
                        kPersistentGrid,
                        {4 * kPersistentGrid, 4 * kPersistentGrid},
                        {2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid},
@@ -1644,7 +1719,6 @@ void init_launch_grids() {
     g.probe[1] = rg(reinterpret_cast<const void*>(k_probe<2, 1024>), 1024, g.probe[1]);
     g.probe[2] = rg(reinterpret_cast<const void*>(k_probe<4, 1024>), 1024, g.probe[2]);
     g.probe[3] = rg(reinterpret_cast<const void*>(k_probe<8, 1024>), 1024, g.probe[3]);
-    g.scan_rows = rg(reinterpret_cast<const void*>(k_scan_rows), kBlock, g.scan_rows);
     g.tile_rows = rg(reinterpret_cast<const void*>(k_tile_rows), kBlock, g.tile_rows);
     g.gather[0] = rg(reinterpret_cast<const void*>(k_gather<false>), kBlock, g.gather[0]);
     g.gather[1] = rg(reinterpret_cast<const void*>(k_gather<true>), kBlock, g.gather[1]);
@@ -1682,10 +1756,6 @@ void launch_scan_tiles(const StepArgs& a, hipStream_t s) {
   WK_LAUNCH(k_scan_tiles, dim3(1), dim3(1024), 0, s, a);
 }
 
-void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
-  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : g_grids.scan_rows;
-  WK_LAUNCH(k_scan_rows, dim3(grid), dim3(kBlock), 0, s, a);
-}
 
 void launch_scan_queries(const StepArgs& a, hipStream_t s) {
   WK_LAUNCH(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
@@ -1701,7 +1771,6 @@ void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
 
 void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s) {
   launch_scan_tiles(a, s);
-  launch_scan_rows(a, 0, s);
   if (a.nq > uint32_t(kBlock) && a.qagg) {
     launch_scan_queries_mb(a, s);  // output tiles are marked by its first launch
     launch_gather_only(a, rows_per_wave, 0, s);
@@ -1711,9 +1780,8 @@ void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s) {
   }
 }
 
-void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s) {
+void launch_scan(const StepArgs& a, hipStream_t s) {
   launch_scan_tiles(a, s);
-  launch_scan_rows(a, rows_hint, s);
   launch_scan_queries(a, s);
 }
 
@@ -1870,10 +1938,11 @@ void launch_dense_build(const Slot* karena, const uint32_t* ktab, uint32_t kshif
                      dim3(kBlock), 0, st, karena, ktab, kshift, s, dense);
 }
 
-void launch_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C, uint64_t T, uint32_t n,
-                        uint64_t* out, hipStream_t s) {
+void launch_window_rows(const uint64_t* tile_sum, const uint64_t* lpfx, uint64_t R, uint64_t j0, uint64_t C,
+                        uint64_t T, uint32_t n, uint64_t* out, hipStream_t s) {
   if (n == 0) return;
-  WK_LAUNCH(k_window_rows, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pfx, R, j0, C, T, n, out);
+  WK_LAUNCH(k_window_rows, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tile_sum, lpfx, R, j0, C, T, n,
+            out);
 }
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {

@@ -2,10 +2,13 @@
 // host-side launch wrappers of the expansion kernels.
 //
 // One pattern step of one or many queries runs as:
-//   probe        -> per input row: edge count (k2u) or keep flag (k2k/k2c)
-//   scan_tiles   -> exclusive scan of per-tile sums
-//   scan_rows    -> per-row 64-bit output offsets + output-tile -> row map
-//   scan_queries -> per-query output rows/offsets, next step's row space
+//   probe        -> per input row: edge count (k2u) or keep flag (k2k/k2c),
+//                   kept as the output rows before each row within its
+//                   kTile-row tile, and each tile's sum
+//   scan_tiles   -> exclusive scan of the per-tile sums; a row's output
+//                   offset is its tile's prefix plus its prefix in the tile
+//   scan_queries -> per-query output rows/offsets, next step's row space,
+//                   output-tile -> row map
 //   gather       -> write the next history table(s)
 // All sizes live in device memory, so a multi-step batch can be captured in a
 // hipGraph with no host round trip.
@@ -107,11 +110,12 @@ struct StepArgs {
   uint64_t* row_start;     // [nq+1] flattened row starts of this step
   uint64_t* next_row_start;// [nq+1] of the next step or null
   uint32_t nq;
-  uint32_t* cnt;           // [rcap]
   uint32_t* eoff;          // [rcap]
-  uint64_t* pfx;           // [rcap+1]
+  uint64_t* lpfx;          // [rcap+1] output rows before each row within its kTile-row tile; entry R:
+                           // the rows of the tile holding row R before it
   uint64_t rcap;
-  uint64_t* tile_sum;      // [tile_cap]
+  uint64_t* tile_sum;      // [tile_cap] output rows of each tile, then (k_scan_tiles) before each tile;
+                           // entry ceil(R / kTile): the step's total
   uint64_t tile_cap;
   uint64_t* tile_row;      // [otile_cap+1] first source row of each output tile
   uint64_t otile_cap;
@@ -131,21 +135,22 @@ struct StepArgs {
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kRowsPerThread = 4;
 constexpr int kTile = kBlock * kRowsPerThread;  // rows per probe/scan tile
+constexpr int kTileShift = 10;
+static_assert((1 << kTileShift) == kTile, "kTile is 2^kTileShift rows");
 constexpr int kOutTile = 4 * kBlock;            // output rows per gather tile (4 per thread)
 
 // Launch wrappers (all on `stream`; grids sized from host hints, kernels
 // re-read the device sizes and stride).  `rows_hint`/`out_hint` = 0 means
 // unknown on the host: a fixed persistent grid is used.
 void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s);
-// scan = scan_tiles + scan_rows + scan_queries
-// scan_tiles, scan_rows, the query scan (multi-block for more than kBlock
-// queries, which also marks output tiles) and the gather of one step with no
-// host size hints: the batched and served paths.
+// scan = scan_tiles + scan_queries
+// scan_tiles, the query scan (multi-block for more than kBlock queries, which
+// also marks output tiles) and the gather of one step with no host size
+// hints: the batched and served paths.
 void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s);
 void launch_scan_queries_mb(const StepArgs& a, hipStream_t s);
-void launch_scan(const StepArgs& a, uint64_t rows_hint, hipStream_t s);
+void launch_scan(const StepArgs& a, hipStream_t s);
 void launch_scan_tiles(const StepArgs& a, hipStream_t s);
-void launch_scan_rows(const StepArgs& a, uint64_t rows_hint, hipStream_t s);
 void launch_scan_queries(const StepArgs& a, hipStream_t s);
 // gather = tile_rows (output-balanced mapping only) + the gather kernel
 void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
@@ -210,10 +215,11 @@ void launch_dense_build(const Slot* karena, const uint32_t* ktab, uint32_t kshif
 // Chunk boundaries of a step run in output windows of C rows: for chunk i
 // (output rows [j, min(T, j + C)) with j = j0 + i * C), out[3i] and
 // out[3i + 1] are the input rows holding its first and last output row and
-// out[3i + 2] the prefix at the first.  pfx holds the step's R + 1 prefixes.
+// out[3i + 2] the prefix at the first.  The step's R + 1 prefixes are
+// tile_sum[g / kTile] + lpfx[g], as k_scan_tiles leaves them.
 constexpr uint32_t kWinGroup = 4096;
-void launch_window_rows(const uint64_t* pfx, uint64_t R, uint64_t j0, uint64_t C, uint64_t T, uint32_t n,
-                        uint64_t* out, hipStream_t s);
+void launch_window_rows(const uint64_t* tile_sum, const uint64_t* lpfx, uint64_t R, uint64_t j0, uint64_t C,
+                        uint64_t T, uint32_t n, uint64_t* out, hipStream_t s);
 
 // One non-empty table for the checksum kernel: `rows` rows of `cols`
 // elements at element offset `off`; rows of all tables numbered from `row0`.
