@@ -1183,6 +1183,35 @@ __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uin
   }
 }
 
+// What the gather needs of a query that owns a whole output tile.
+struct GatherQuery {
+  uint32_t q;
+  bool nospace, expand, ck;
+  uint32_t ic, oc;
+  uint64_t r0, p0;
+  uint32_t* dbase;
+  const uint32_t* sbase;
+  int64_t voff;      // the expanded segment's edge addressing (DevSeg)
+  uint32_t vb_base;
+
+  __device__ __forceinline__ void load(const StepArgs& a, uint32_t qi) {
+    const StepDesc& d = a.desc[qi];
+    q = qi;
+    nospace = d.out_off == kNoSpace;
+    expand = d.op == OP_EXPAND;
+    ck = a.ck && d.final_step;
+    ic = d.in_cols;
+    oc = d.out_cols;
+    r0 = a.row_start[qi];
+    p0 = pfx_at(a, r0);
+    dbase = (d.final_step ? a.arena : a.tout) + d.out_off;
+    sbase = a.tin + d.in_off;
+    const DevSeg& s = a.segs[expand ? d.seg : 0];
+    voff = s.voff;
+    vb_base = s.vb_base;
+  }
+};
+
 template <bool CK>
 __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   if (a.st->status) return;
@@ -1198,6 +1227,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   const uint64_t R = step_rows(a);
   const uint64_t total = out_end(a);
   const uint64_t nt = out_tiles(a);
+  // the last query that owned a whole tile of this workgroup: its descriptor
+  // fields, row start, first output row and segment, kept in scalar
+  // registers while the next tiles belong to it too (a one-query step, or a
+  // query larger than a tile)
+  GatherQuery gq;
+  gq.q = ~0u;
   for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
     const uint64_t j0 = a.win_lo + t * kOutTile;
     // this thread's index, opaque to the compiler, so that the values derived
@@ -1260,21 +1295,25 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
       s_ck[tid] = 0;
     }
     __syncthreads();
-    if (staged && qhi - qlo == 1 && a.desc[qlo].out_off == kNoSpace) {
+    if (staged && qhi - qlo == 1) {
+      // one query owns the whole tile: its descriptor, row start and segment
+      // come from scalar registers, loaded when the workgroup's query changes
+      const uint32_t q = __builtin_amdgcn_readfirstlane(qlo);
+      if (q != gq.q) gq.load(a, q);
+    }
+    if (staged && qhi - qlo == 1 && gq.nospace) {
       // the tile's one query lost its final table (no room in the arena)
     } else if (staged && qhi - qlo == 1) {
-      // one query owns the whole tile: its descriptor, row start and segment
-      // are loaded once, into scalar registers
-      const uint32_t q = __builtin_amdgcn_readfirstlane(qlo);
-      const StepDesc& d = a.desc[q];
-      const uint64_t r0 = a.row_start[q];
-      const uint64_t p0 = pfx_at(a, r0);
-      const uint32_t ic = d.in_cols, oc = d.out_cols;
-      const bool expand = d.op == OP_EXPAND;
-      uint32_t* const dbase = (d.final_step ? a.arena : a.tout) + d.out_off;
-      const uint32_t* const sbase = a.tin + d.in_off;
-      const DevSeg sg = a.segs[expand ? d.seg : 0];
-      const bool ck = CK && a.ck && d.final_step;
+      const uint32_t q = gq.q;
+      const uint64_t r0 = gq.r0, p0 = gq.p0;
+      const uint32_t ic = gq.ic, oc = gq.oc;
+      const bool expand = gq.expand;
+      uint32_t* const dbase = gq.dbase;
+      const uint32_t* const sbase = gq.sbase;
+      DevSeg sg;  // the fields edge_at reads
+      sg.voff = gq.voff;
+      sg.vb_base = gq.vb_base;
+      const bool ck = CK && gq.ck;
       uint64_t hsum = 0;  // row hashes of this thread's final rows
       constexpr int kR = kOutTile / kBlock;
       // each output row's source row in the tile, by independent searches
@@ -1793,8 +1832,10 @@ void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s) {
 
 void launch_gather_only(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s) {
   if (rows_per_wave <= 0) {
-    const unsigned grid =
-        out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, 1u << 22) : g_grids.gather[a.ck ? 1 : 0];
+    // the workgroups resident at once stride over the tiles (fewer when the
+    // host knows the step has fewer tiles)
+    const unsigned resident = g_grids.gather[a.ck ? 1 : 0];
+    const unsigned grid = out_hint ? grid_for((out_hint + kOutTile - 1) / kOutTile, resident) : resident;
     // the checksum variant only when final tables are checksummed (serving)
     if (a.ck)
       WK_LAUNCH(k_gather<true>, dim3(grid), dim3(kBlock), 0, s, a);
