@@ -528,13 +528,110 @@ __device__ __forceinline__ void probe_row2(const StepArgs& a, const ProbeDesc& d
   finish_row<W>(a, d, s, row1, h1 && (!tix || kv1 < (1u << NBITS_IDX)), p1, c1, eo1);
 }
 
-// One block of NT threads per tile of kTile rows, in kTile / (NT / W)
+// The rows of tile [tlo, thi), all of one query (descriptor d, segment s,
+// input rows from tbase, first row r0): counts into s_cnt, edge offsets into
+// eoff.  The W lanes of a group share a row.
+template <int W, int NT>
+__device__ __forceinline__ void probe_tile_one(const StepArgs& a, const ProbeDesc& d, const DevSeg& s,
+                                               const uint32_t* tbase, uint64_t r0, uint64_t tlo, uint64_t thi,
+                                               uint32_t* s_cnt) {
+  constexpr int G = NT / W;
+  constexpr int kPasses = kTile / G;
+  const int grp = threadIdx.x / W, gl = threadIdx.x % W;
+  const bool live = d.op != OP_SKIP && d.seg >= 0;
+  const bool tb = d.op == OP_FILTER_TYPE;
+  if constexpr (kPasses % 2 == 0) {
+    // rows g and g + G of each pair of passes together
+    const bool res = live && seg_resident(s);
+    for (int pass = 0; pass < kPasses; pass += 2) {
+      const uint64_t g = tlo + uint64_t(pass) * G + grp;
+      if (g < thi) {
+        const uint64_t g1 = g + G;
+        const bool has1 = g1 < thi;
+        const uint64_t gr1 = has1 ? g1 : g;  // a lone last row probes itself twice
+        uint32_t c = 0, eo = 0, c1 = 0, eo1 = 0;
+        if (tb) {
+          c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
+          c1 = type_bit(a, d, tbase + (gr1 - r0) * d.in_cols);
+        } else if (res) {
+          probe_row2<W>(a, d, s, tbase + (g - r0) * d.in_cols, tbase + (gr1 - r0) * d.in_cols, c, eo, c1, eo1);
+        }
+        if (gl == 0) {
+          s_cnt[g - tlo] = c;
+          a.eoff[g] = eo;
+          if (has1) {
+            s_cnt[g1 - tlo] = c1;
+            a.eoff[g1] = eo1;
+          }
+        }
+      }
+    }
+  } else {
+    for (int pass = 0; pass < kPasses; ++pass) {
+      const uint64_t g = tlo + uint64_t(pass) * G + grp;
+      if (g < thi) {
+        uint32_t c = 0, eo = 0;
+        if (tb) c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
+        else if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
+        if (gl == 0) {
+          s_cnt[g - tlo] = c;
+          a.eoff[g] = eo;
+        }
+      }
+    }
+  }
+}
+
+// The end of a tile: the prefix of its counts within it (kTile / NT
+// consecutive rows per thread) to lpfx, and its sum to tile_sum[t].
+template <int NT>
+__device__ __forceinline__ void probe_tile_end(const StepArgs& a, uint64_t t, uint64_t tlo, uint64_t thi,
+                                               const uint32_t* s_cnt, uint64_t* s_pfx) {
+  constexpr int kPer = kTile / NT;
+  // the tile's prefix of counts, kPer consecutive rows per thread, and its
+  // sum
+  __syncthreads();
+  const uint32_t nrow = uint32_t(thi - tlo);
+  // this thread's index, opaque to the compiler, so that the addresses below
+  // are formed here and not kept in registers across the whole tile loop
+  uint32_t tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  uint32_t cc[kPer];  // a row's count is below 2^28
+  uint64_t cs = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t i = tid * kPer + k;
+    cc[k] = i < nrow ? s_cnt[i] : 0u;
+    cs += cc[k];
+  }
+  uint64_t tsum = 0;
+  uint64_t ex = block_excl_scan<NT>(cs, &tsum);  // ends with a barrier: s_cnt is free again
+  if constexpr (kPer == 1) {
+    if (tid < nrow) a.lpfx[tlo + tid] = ex;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      s_pfx[tid * kPer + k] = ex;
+      ex += cc[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t i = k * NT + tid;
+      if (i < nrow) a.lpfx[tlo + i] = s_pfx[i];
+    }
+    __syncthreads();  // before the next tile's prefixes overwrite s_pfx
+  }
+  if (threadIdx.x == 0) a.tile_sum[t] = tsum;
+}
+
+// One workgroup of NT threads per tile of kTile rows, in kTile / (NT / W)
 // passes.  A tile inside one query reads the descriptor and segment once,
 // into scalar registers; a tile spanning several queries stages their row
 // starts in LDS and each row finds its query there.  NT = 1024 (one row per
 // lane at W = 1) serves batches, whose tiles mix queries; NT = 256 serves one
-// large query, where a smaller workgroup lets a finished one be replaced
-// sooner.  (Capping the registers for one more wave per SIMD made the large
+// query, whose descriptor and segment each workgroup reads once for all its
+// tiles.  (Capping the registers for one more wave per SIMD made the large
 // query's probe 30% slower: fewer loads stay in flight per wave.)
 template <int W, int NT>
 __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
@@ -557,53 +654,84 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
   constexpr int G = NT / W;
   constexpr int kPasses = kTile / G;
   const int grp = threadIdx.x / W, gl = threadIdx.x % W;
-  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint64_t tlo = t * kTile, thi = (tlo + kTile < R) ? tlo + kTile : R;
-    uint32_t q0, q1;
-    block_find_queries<NT>(a.row_start, a.nq, tlo, thi - 1, q0, q1);
-    if (q0 == q1) {
-      // one query owns the tile: descriptor, segment and row base once, in
-      // scalar registers
-      const uint32_t q = __builtin_amdgcn_readfirstlane(q0);
-      const ProbeDesc d = read_probe_desc(a.desc + q);
-      const bool live = d.op != OP_SKIP && d.seg >= 0;
-      const bool tb = d.op == OP_FILTER_TYPE;
-      const DevSeg s = a.segs[live ? d.seg : 0];
-      const uint32_t* const tbase = a.tin + a.desc[q].in_off;
-      const uint64_t r0 = a.row_start[q];
-      if constexpr (kPasses % 2 == 0) {
-        // rows g and g + G of each pair of passes together
-        const bool res = live && seg_resident(s);
-        for (int pass = 0; pass < kPasses; pass += 2) {
-          const uint64_t g = tlo + uint64_t(pass) * G + grp;
-          if (g < thi) {
-            const uint64_t g1 = g + G;
-            const bool has1 = g1 < thi;
-            const uint64_t gr1 = has1 ? g1 : g;  // a lone last row probes itself twice
-            uint32_t c = 0, eo = 0, c1 = 0, eo1 = 0;
-            if (tb) {
-              c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
-              c1 = type_bit(a, d, tbase + (gr1 - r0) * d.in_cols);
-            } else if (res) {
-              probe_row2<W>(a, d, s, tbase + (g - r0) * d.in_cols, tbase + (gr1 - r0) * d.in_cols, c, eo, c1, eo1);
-            }
-            if (gl == 0) {
-              s_cnt[g - tlo] = c;
-              a.eoff[g] = eo;
-              if (has1) {
-                s_cnt[g1 - tlo] = c1;
-                a.eoff[g1] = eo1;
-              }
-            }
-          }
-        }
+  if constexpr (NT != 1024) {
+    // one query (launch_probe uses NT = 1024 for every launch of several):
+    // its descriptor, segment and row base are read from memory once per
+    // workgroup, into LDS, and from there for each tile
+    __shared__ ProbeDesc s_d1[1];
+    __shared__ DevSeg s_s1[1];
+    __shared__ uint64_t s_x1[2];
+    if (threadIdx.x == 0) {
+      const ProbeDesc d = read_probe_desc(a.desc);
+      s_d1[0] = d;
+      s_s1[0] = a.segs[d.op != OP_SKIP && d.seg >= 0 ? d.seg : 0];
+      s_x1[0] = a.desc[0].in_off;
+      s_x1[1] = a.row_start[0];
+    }
+    __syncthreads();
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      const uint64_t tlo = t * kTile, thi = (tlo + kTile < R) ? tlo + kTile : R;
+      // an index of 0 the compiler cannot see through, so that the LDS reads
+      // happen per tile and their values are not held across the loop
+      uint32_t z = 0;
+      asm volatile("" : "+s"(z));
+      const ProbeDesc d = s_d1[z];
+      probe_tile_one<W, NT>(a, d, s_s1[z], a.tin + s_x1[z], s_x1[z + 1], tlo, thi, s_cnt);
+      probe_tile_end<NT>(a, t, tlo, thi, s_cnt, s_pfx);
+    }
+  } else {
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      const uint64_t tlo = t * kTile, thi = (tlo + kTile < R) ? tlo + kTile : R;
+      uint32_t q0, q1;
+      block_find_queries<NT>(a.row_start, a.nq, tlo, thi - 1, q0, q1);
+      if (q0 == q1) {
+        // one query owns the tile: descriptor, segment and row base once, in
+        // scalar registers
+        const uint32_t q = __builtin_amdgcn_readfirstlane(q0);
+        const ProbeDesc d = read_probe_desc(a.desc + q);
+        const DevSeg s = a.segs[d.op != OP_SKIP && d.seg >= 0 ? d.seg : 0];
+        probe_tile_one<W, NT>(a, d, s, a.tin + a.desc[q].in_off, a.row_start[q], tlo, thi, s_cnt);
       } else {
+        const uint32_t nqt = q1 - q0 + 1;
+        const bool staged = QS > 1 && nqt <= uint32_t(QS);
+        if (staged && threadIdx.x < nqt) {
+          // the tile's queries: row start, probe descriptor, input offset and
+          // segment, so that a row's chain starts from LDS instead of three
+          // dependent global loads
+          const uint32_t q = q0 + threadIdx.x;
+          s_rs[threadIdx.x] = a.row_start[q];
+          const ProbeDesc d = read_probe_desc(a.desc + q);
+          s_pd[threadIdx.x] = d;
+          s_in[threadIdx.x] = a.desc[q].in_off;
+          if (d.op != OP_SKIP && d.seg >= 0) s_sg[threadIdx.x] = a.segs[d.seg];
+        }
+        __syncthreads();
         for (int pass = 0; pass < kPasses; ++pass) {
           const uint64_t g = tlo + uint64_t(pass) * G + grp;
           if (g < thi) {
             uint32_t c = 0, eo = 0;
-            if (tb) c = type_bit(a, d, tbase + (g - r0) * d.in_cols);
-            else if (live) probe_row<W>(a, d, s, tbase + (g - r0) * d.in_cols, c, eo);
+            // each branch probes by itself, so only values leave LDS or memory
+            if (staged) {
+              uint32_t l = 0, h = nqt;
+              while (h - l > 1) {
+                const uint32_t m = (l + h) >> 1;
+                if (s_rs[m] <= g) l = m;
+                else h = m;
+              }
+              const ProbeDesc d = s_pd[l];
+              if (d.op == OP_FILTER_TYPE)
+                c = type_bit(a, d, a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols);
+              else if (d.op != OP_SKIP && d.seg >= 0)
+                probe_row<W>(a, d, s_sg[l], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
+            } else {
+              const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
+              const ProbeDesc d = read_probe_desc(a.desc + q);
+              if (d.op == OP_FILTER_TYPE)
+                c = type_bit(a, d, a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols);
+              else if (d.op != OP_SKIP && d.seg >= 0)
+                probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
+                             eo);
+            }
             if (gl == 0) {
               s_cnt[g - tlo] = c;
               a.eoff[g] = eo;
@@ -611,89 +739,8 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
           }
         }
       }
-    } else {
-      const uint32_t nqt = q1 - q0 + 1;
-      const bool staged = QS > 1 && nqt <= uint32_t(QS);
-      if (staged && threadIdx.x < nqt) {
-        // the tile's queries: row start, probe descriptor, input offset and
-        // segment, so that a row's chain starts from LDS instead of three
-        // dependent global loads
-        const uint32_t q = q0 + threadIdx.x;
-        s_rs[threadIdx.x] = a.row_start[q];
-        const ProbeDesc d = read_probe_desc(a.desc + q);
-        s_pd[threadIdx.x] = d;
-        s_in[threadIdx.x] = a.desc[q].in_off;
-        if (d.op != OP_SKIP && d.seg >= 0) s_sg[threadIdx.x] = a.segs[d.seg];
-      }
-      __syncthreads();
-      for (int pass = 0; pass < kPasses; ++pass) {
-        const uint64_t g = tlo + uint64_t(pass) * G + grp;
-        if (g < thi) {
-          uint32_t c = 0, eo = 0;
-          // each branch probes by itself, so only values leave LDS or memory
-          if (staged) {
-            uint32_t l = 0, h = nqt;
-            while (h - l > 1) {
-              const uint32_t m = (l + h) >> 1;
-              if (s_rs[m] <= g) l = m;
-              else h = m;
-            }
-            const ProbeDesc d = s_pd[l];
-            if (d.op == OP_FILTER_TYPE)
-              c = type_bit(a, d, a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols);
-            else if (d.op != OP_SKIP && d.seg >= 0)
-              probe_row<W>(a, d, s_sg[l], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
-          } else {
-            const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
-            const ProbeDesc d = read_probe_desc(a.desc + q);
-            if (d.op == OP_FILTER_TYPE)
-              c = type_bit(a, d, a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols);
-            else if (d.op != OP_SKIP && d.seg >= 0)
-              probe_row<W>(a, d, a.segs[d.seg], a.tin + a.desc[q].in_off + (g - a.row_start[q]) * d.in_cols, c,
-                           eo);
-          }
-          if (gl == 0) {
-            s_cnt[g - tlo] = c;
-            a.eoff[g] = eo;
-          }
-        }
-      }
+      probe_tile_end<NT>(a, t, tlo, thi, s_cnt, s_pfx);
     }
-    // the tile's prefix of counts, kPer consecutive rows per thread, and its
-    // sum
-    __syncthreads();
-    const uint32_t nrow = uint32_t(thi - tlo);
-    // this thread's index, opaque to the compiler, so that the addresses below
-    // are formed here and not kept in registers across the whole tile loop
-    uint32_t tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    uint32_t cc[kPer];  // a row's count is below 2^28
-    uint64_t cs = 0;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint32_t i = tid * kPer + k;
-      cc[k] = i < nrow ? s_cnt[i] : 0u;
-      cs += cc[k];
-    }
-    uint64_t tsum = 0;
-    uint64_t ex = block_excl_scan<NT>(cs, &tsum);  // ends with a barrier: s_cnt is free again
-    if constexpr (kPer == 1) {
-      if (tid < nrow) a.lpfx[tlo + tid] = ex;
-    } else {
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        s_pfx[tid * kPer + k] = ex;
-        ex += cc[k];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const uint32_t i = k * NT + tid;
-        if (i < nrow) a.lpfx[tlo + i] = s_pfx[i];
-      }
-      __syncthreads();  // before the next tile's prefixes overwrite s_pfx
-    }
-    if (threadIdx.x == 0) a.tile_sum[t] = tsum;
   }
 }
 
@@ -1721,12 +1768,14 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
 // workgroup holds 120 KB of LDS, one per CU, so 2,048 of them took 8 rounds.
 struct LaunchGrids {
   unsigned probe[4];       // k_probe<W, 1024>, W = 1, 2, 4, 8
+  unsigned probe1q[4];     // k_probe<W, 256>, the one-query probe
   unsigned tile_rows;
   unsigned gather[2];      // k_gather<false>, k_gather<true>
   unsigned gather_rows[4]; // k_gather_rows<1, 4, 16, 64>
   unsigned query_sums;
 };
 LaunchGrids g_grids = {{kPersistentGrid, kPersistentGrid, kPersistentGrid, kPersistentGrid},
+                       {4 * kPersistentGrid, 4 * kPersistentGrid, 4 * kPersistentGrid, 4 * kPersistentGrid},
                        kPersistentGrid,
                        {4 * kPersistentGrid, 4 * kPersistentGrid},
                        {2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid},
@@ -1758,6 +1807,10 @@ void init_launch_grids() {
     g.probe[1] = rg(reinterpret_cast<const void*>(k_probe<2, 1024>), 1024, g.probe[1]);
     g.probe[2] = rg(reinterpret_cast<const void*>(k_probe<4, 1024>), 1024, g.probe[2]);
     g.probe[3] = rg(reinterpret_cast<const void*>(k_probe<8, 1024>), 1024, g.probe[3]);
+    g.probe1q[0] = rg(reinterpret_cast<const void*>(k_probe<1, 256>), 256, g.probe1q[0]);
+    g.probe1q[1] = rg(reinterpret_cast<const void*>(k_probe<2, 256>), 256, g.probe1q[1]);
+    g.probe1q[2] = rg(reinterpret_cast<const void*>(k_probe<4, 256>), 256, g.probe1q[2]);
+    g.probe1q[3] = rg(reinterpret_cast<const void*>(k_probe<8, 256>), 256, g.probe1q[3]);
     g.tile_rows = rg(reinterpret_cast<const void*>(k_tile_rows), kBlock, g.tile_rows);
     g.gather[0] = rg(reinterpret_cast<const void*>(k_gather<false>), kBlock, g.gather[0]);
     g.gather[1] = rg(reinterpret_cast<const void*>(k_gather<true>), kBlock, g.gather[1]);
@@ -1780,7 +1833,10 @@ void probe_launch(const StepArgs& a, unsigned grid, bool one_query, hipStream_t 
 
 void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
   const int wi = w == 8 ? 3 : w == 4 ? 2 : w == 2 ? 1 : 0;
-  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, 1u << 20) : g_grids.probe[wi];
+  // the workgroups resident at once stride over the tiles (fewer when the
+  // host knows the step has fewer tiles)
+  const unsigned resident = a.nq == 1 ? g_grids.probe1q[wi] : g_grids.probe[wi];
+  const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, resident) : resident;
   // w lanes share one key: each reads 8/w slots of the 128-B bucket, so a
   // wave's loads of one slot index cover 64/w buckets, each line coalesced
   switch (w) {
