@@ -1303,8 +1303,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
     const uint64_t S = staged ? 1 : (nr + kOutTile - 1) / kOutTile;
     const uint32_t ns = uint32_t(staged ? nr : (nr + S - 1) / S);
     // unconditional loads at clamped indices (rows lo..hi-1 exist), so all
-    // of them are in flight together
-    if (staged) {
+    // of them are in flight together; a tile of at most kBlock source rows
+    // (a heavy expansion's) takes one load of each per thread
+    if (staged && nr <= uint64_t(kBlock)) {
+      const uint64_t gi = lo + (tid < uint32_t(nr) ? tid : uint32_t(nr) - 1);
+      const uint64_t pv = pfx_at(a, gi);
+      const uint32_t ev = a.eoff[gi];
+      if (tid < uint32_t(nr)) {
+        const int64_t r = int64_t(pv - j0);
+        s_rel[tid] = int32_t(r < (int64_t(1) << 30) ? r : (int64_t(1) << 30));
+        s_eoff[tid] = ev;
+      }
+    } else if (staged) {
 #pragma unroll
       for (int k = 0; k < (kStage + kBlock - 1) / kBlock; ++k) {
         const uint32_t i = tid + k * kBlock;
@@ -1393,22 +1403,32 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
           e[k] = (expand && act[k]) ? edge_at(a, sg, uint64_t(s_eoff[ls[k]]) + uint64_t(int64_t(jl) - s_rel[ls[k]]))
                                     : 0u;
         }
+        // two-column rows at an 8-byte aligned table go out as one 8-byte
+        // store each (rows are 8 bytes apart, so the table's alignment decides;
+        // not in the checksum variant, whose registers it would cost a wave)
+        const bool pair = !CK && oc == 2 && ic + (expand ? 1u : 0u) == 2 && (reinterpret_cast<uintptr_t>(dbase) & 7) == 0;
 #pragma unroll
         for (int k = 0; k < kR; ++k) {
           if (!act[k]) continue;
           const uint32_t jl = k * kBlock + tid;
           uint32_t* dst = dbase + (j0 + jl - p0 - a.win_lo) * oc;
           uint64_t rh = row_hash_init(oc);
+          if (pair) {
+            const uint32_t w1 = expand ? e[k] : v[k][1];
+            *reinterpret_cast<uint2*>(dst) = make_uint2(v[k][0], w1);
+            if (ck) rh = row_hash_step(row_hash_step(rh, v[k][0], 0), w1, 1);
+          } else {
 #pragma unroll
-          for (int c = 0; c < kRegCols; ++c) {
-            if (uint32_t(c) < ic) {
-              dst[c] = v[k][c];
-              if (ck) rh = row_hash_step(rh, v[k][c], c);
+            for (int c = 0; c < kRegCols; ++c) {
+              if (uint32_t(c) < ic) {
+                dst[c] = v[k][c];
+                if (ck) rh = row_hash_step(rh, v[k][c], c);
+              }
             }
-          }
-          if (expand) {
-            dst[ic] = e[k];
-            if (ck) rh = row_hash_step(rh, e[k], ic);
+            if (expand) {
+              dst[ic] = e[k];
+              if (ck) rh = row_hash_step(rh, e[k], ic);
+            }
           }
           if (ck) hsum += rh;
         }
