@@ -82,6 +82,14 @@ def parse_args():
     ap.add_argument("--gstore-budget-gb", type=float, default=0, help="0: the whole store")
     ap.add_argument("--table-budget-gb", type=float, default=48)
     ap.add_argument("--rates", type=str, default="", help="serving offered rates, q/s, comma separated")
+    ap.add_argument("--mix", type=str, default="",
+                    help="serving stream as LIGHT/HEAVY percent, e.g. 95/5: light queries have at most "
+                         "--light-rows result rows, heavy ones at least --heavy-rows (classified by the batched "
+                         "run's row counts, drawn from the first --mix-slices timed slices); default: the stream "
+                         "as generated")
+    ap.add_argument("--light-rows", type=int, default=100)
+    ap.add_argument("--heavy-rows", type=int, default=1_000_000)
+    ap.add_argument("--mix-slices", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--heavy-footprint", type=int, default=0,
                     help="serving: projected table bytes from which a query runs on the second lane "
@@ -328,10 +336,10 @@ def cpu_baseline_and_gate(args, wg, dev, tri, single, slices, tq, rows_t, st_t, 
         k += 1
     served = 0
     for s in serve.values():
-        for q in range(min(s["_n"], n)):
-            if s["_st"][q] == 0:
+        for i, q in enumerate(s["_idx"]):
+            if s["_st"][i] == 0 and int(q) in ck_of:
                 served += 1
-                if (int(s["_ck"][q]), int(s["_rows"][q])) != ck_of[q]:
+                if (int(s["_ck"][i]), int(s["_rows"][i])) != ck_of[int(q)]:
                     cks_ok = False
     gate["stream_rows"] = bool(counts_ok)
     gate["stream_fingerprints"] = bool(cks_ok)
@@ -560,15 +568,27 @@ def main():
 
     def run_fetch_pipelined(ks):
         """run_pipelined with each slice's arena copied to the host after its
-        wait (while the next slice runs), and the arenas of further rounds
-        after each round."""
-        pend, nbytes = [], [0]
+        wait (while the next slice runs).  The queries whose tables did not
+        fit a slice's arena run in further synchronous rounds, each arena
+        copied after its round, once every pipelined slice is fetched: a
+        synchronous round reuses the first lane's arena, which may hold a
+        pipelined slice not yet fetched."""
+        pend, nbytes, st_of = [], [0], {}
 
         def complete(k, tk):
             dev.batch_wait(tk)
             nbytes[0] += to_host(lambda d, c, u: L.wk_batch_fetch(dev._h, tk, d, c, u))
+            st_of[k] = slices[k].status.copy()
+
+        for k in ks:
+            pend.append((k, dev.batch_submit(slices[k], use_graph=not args.no_graph)))
+            if len(pend) == inflight:
+                complete(*pend.pop(0))
+        for kt in pend:
+            complete(*kt)
+        for k in ks:
             base = k * args.nq
-            st = slices[k].status.copy()
+            st = st_of[k]
             idx = np.arange(args.nq)
             while True:
                 fail = idx[st[idx] == wg.WK_E_NOMEM_BUDGET]
@@ -579,13 +599,6 @@ def main():
                 nbytes[0] += to_host(lambda d, c, u: L.wk_batch_download(dev._h, d, c, u))
                 st[fail] = s2
                 idx = fail
-
-        for k in ks:
-            pend.append((k, dev.batch_submit(slices[k], use_graph=not args.no_graph)))
-            if len(pend) == inflight:
-                complete(*pend.pop(0))
-        for kt in pend:
-            complete(*kt)
         return nbytes[0]
 
     run_fetch_pipelined(range(args.warmup))
@@ -623,6 +636,23 @@ def main():
 
     # ---------------------------------------------------------------- serve
     serve = {}
+    mix_info = None
+    if not args.no_serve and args.mix:
+        # the served stream in the given light/heavy proportion, from the
+        # first timed slices' queries as the batched run classified them
+        pl, ph = (float(x) for x in args.mix.split("/"))
+        span = min(len(tq), args.mix_slices * args.nq)
+        ok_t = st_t[:span] == 0
+        light = np.flatnonzero(ok_t & (rows_t[:span] <= args.light_rows))
+        heavy = np.flatnonzero(ok_t & (rows_t[:span] >= args.heavy_rows))
+        if len(light) == 0 or len(heavy) == 0:
+            raise SystemExit("--mix: %d light and %d heavy queries in the first %d timed queries" % (
+                len(light), len(heavy), span))
+        mix_info = {"light_pct": pl, "heavy_pct": ph, "light_rows_max": args.light_rows,
+                    "heavy_rows_min": args.heavy_rows, "light_candidates": int(len(light)),
+                    "heavy_candidates": int(len(heavy)), "drawn_from_first_queries": int(span)}
+        log("serving mix %s: %d light (<= %d rows) and %d heavy (>= %d rows) candidates in the first %d queries" % (
+            args.mix, len(light), args.light_rows, len(heavy), args.heavy_rows, span))
     if not args.no_serve:
         rng = np.random.default_rng(args.seed + 7)
         if args.rates:
@@ -631,12 +661,21 @@ def main():
         else:
             plan = [("per_query", 0.7 * qps_perq), ("batched", 10 * 0.7 * qps_perq)]
         for mode, rate in plan:
-            n = min(len(tq), int(rate * 2.0) + 256)
+            if mix_info:
+                n = int(rate * 2.0) + 256
+                mrng = np.random.default_rng(args.seed + 11)
+                is_heavy = mrng.random(n) < ph / (pl + ph)
+                idx = np.where(is_heavy, heavy[mrng.integers(len(heavy), size=n)],
+                               light[mrng.integers(len(light), size=n)])
+            else:
+                n = min(len(tq), int(rate * 2.0) + 256)
+                idx = np.arange(n)
+            sq = [tq[int(i)] for i in idx]
             arr = np.cumsum(rng.exponential(1.0 / rate, size=n))
             dev.reset_kernel_stats()
             # only the continuous-batching loop keeps its (host-side) iteration timer
             dev.set_profile(mode != "per_query")
-            r = dev.serve(tq[:n], arr, max_inflight=4096, per_query=(mode == "per_query"),
+            r = dev.serve(sq, arr, max_inflight=4096, per_query=(mode == "per_query"),
                           heavy_footprint=(2**64 - 1) if args.heavy_footprint < 0 else args.heavy_footprint)
             phases = {k[6:]: {"count": v["launches"], "ms": round(v["total_ms"], 3)}
                       for k, v in dev.kernel_stats().items() if k.startswith("serve:")}
@@ -646,7 +685,17 @@ def main():
                            "p50_us": float(np.percentile(lat[ok], 50)) if ok.any() else None,
                            "p99_us": float(np.percentile(lat[ok], 99)) if ok.any() else None,
                            "ok": int(ok.sum()), "iterations": int(r["batches"]), "phases": phases,
-                           "_ck": r["checksum"], "_st": r["status"], "_rows": r["rows"], "_n": n}
+                           "_ck": r["checksum"], "_st": r["status"], "_rows": r["rows"], "_idx": idx}
+            if mix_info:
+                hv = rows_t[idx] >= args.heavy_rows
+                serve[mode]["heavy_share"] = float(np.mean(hv))
+                for cls, m in (("light", ok & ~hv), ("heavy", ok & hv)):
+                    if m.any():
+                        serve[mode][cls] = {"p50_us": float(np.percentile(lat[m], 50)),
+                                            "p99_us": float(np.percentile(lat[m], 99)), "ok": int(m.sum())}
+                log("  light p50/p99 %s / %s us, heavy p50/p99 %s / %s us" % (
+                    *(round(serve[mode].get("light", {}).get(k, -1)) for k in ("p50_us", "p99_us")),
+                    *(round(serve[mode].get("heavy", {}).get(k, -1)) for k in ("p50_us", "p99_us"))))
             log("serve %s @ %.0f q/s offered: %.0f q/s, p50 %s us, p99 %s us, %d ok; %s" % (
                 mode, rate, n / r["wall_s"], serve[mode]["p50_us"], serve[mode]["p99_us"], ok.sum(), phases))
 
@@ -682,7 +731,7 @@ def main():
         for k in ("_table", "_pats", "_nv", "_v2c"):
             r.pop(k, None)
     for r in serve.values():
-        for k in ("_ck", "_st", "_rows", "_n"):
+        for k in ("_ck", "_st", "_rows", "_idx"):
             r.pop(k, None)
 
     out.update({
@@ -694,7 +743,7 @@ def main():
         "per_query_qps": qps_perq, "batched_vs_per_query": qps_batched / qps_perq,
         "single_query": single,
         "edges_per_s": (edges / sum(r["latency_us_mean"] * 1e-6 for r in single.values())) if single else None,
-        "serve": serve, "partition": partition, "final_process": final, "roofline": roofline,
+        "serve": serve, "serve_mix": mix_info, "partition": partition, "final_process": final, "roofline": roofline,
         "cpu_baseline": cpu, "parity_gate": gate,
         "gstore": dict(gstore, h2d_bytes=dev.stats()["h2d_bytes"], segments_evicted=dev.stats()["segments_evicted"]),
     })

@@ -328,12 +328,14 @@ int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* 
  * caller can prepare and submit the next batch while this one runs.  Up to
  * opts.batch_lanes batches (2 by default) can be in flight at once, each on
  * its own lane (stream, history tables, scratch and result arena, allocated
- * with the handle's table budget on a lane's first use); one more submit
- * returns WK_E_STATE.
+ * with the handle's table budget on a lane's first use).  A lane takes a new
+ * batch only once the ticket of its last one has been waited for, so one
+ * more submit returns WK_E_STATE.
  * `out` must stay valid until the batch is waited for: cols and v2c are
  * written by the submit, rows, status and offset by wk_batch_wait, or by any
  * other call on the handle that uses a lane, which first completes the
- * batches in flight.  wk_batch_fetch copies a waited-for ticket's result
+ * batches in flight.  The wait of a batch completed that way returns the
+ * status it completed with.  wk_batch_fetch copies a waited-for ticket's result
  * arena (offsets as in `out`) to the host until its lane starts another
  * batch or another call uses the lanes (WK_E_STATE after that).  Under a
  * gstore budget smaller than the store, a submit first completes the batches

@@ -70,8 +70,9 @@ def test_two_batches_in_flight(univ10_dev, univ10_oracle, univ10_store, use_grap
 
 
 def test_batch_completed_by_another_call(univ10_dev, univ10_oracle, univ10_store):
-    """A call that uses the lanes first completes the batches in flight:
-    their result arrays are filled and their wait returns at once."""
+    """A call that uses a lane first completes the batches in flight: their
+    result arrays are filled and their wait returns at once; only the arena
+    of the lane it reuses is lost."""
     dev = univ10_dev
     sl = _slices(univ10_store, 2, 100, seed=43)
     pbs = [wg.PreparedBatch(s) for s in sl]
@@ -89,11 +90,12 @@ def test_batch_completed_by_another_call(univ10_dev, univ10_oracle, univ10_store
     dev.batch_wait(tb)
     for (pats, nv), r in zip(sl[0] + sl[1], np.concatenate([pbs[0].rows, pbs[1].rows])):
         assert int(r) == univ10_oracle.query(pats, nv)[0].shape[0], pats
-    # the other call may have reused the arenas: the tables are not fetched
-    for t, pb in ((ta, pbs[0]), (tb, pbs[1])):
-        with pytest.raises(wg.WkError) as e:
-            dev.batch_fetch(t, pb)
-        assert e.value.code == wg.WK_E_STATE
+    # the query reused the first lane's arena: ta's tables are not fetched;
+    # tb's lane was not reused
+    with pytest.raises(wg.WkError) as e:
+        dev.batch_fetch(ta, pbs[0])
+    assert e.value.code == wg.WK_E_STATE
+    _check(univ10_oracle, sl[1], dev.batch_fetch(tb, pbs[1]))
 
 
 def test_pipelined_overflow_and_budget_fallbacks(univ10_store, univ10_oracle):
@@ -133,6 +135,46 @@ def test_pipelined_overflow_and_budget_fallbacks(univ10_store, univ10_oracle):
         if k + 2 < 4:
             tickets.append(dev.batch_submit(pbs[k + 2], v2c=True))
     assert dev.stats()["segments_evicted"] > 0
+    dev.close()
+
+
+def test_tickets_completed_by_submits_and_runs_under_swapping(univ10_store, univ10_oracle):
+    """Under a gstore budget smaller than the store a submit first completes
+    the other lane's batch, and a synchronous batch between submit and wait
+    completes the rest and reuses the first lane: every ticket's wait still
+    returns its batch's status, with its result arrays exact, and a lane
+    whose finished batch was not waited for takes no new submit."""
+    info = univ10_store.info()
+    total = info["num_buckets"] * 128 + info["num_values"] * 4
+    dev = wg.Device(univ10_store, gstore_budget=total * 2 // 5, key_block_shift=10, value_block_shift=14,
+                    table_budget=256 << 20, dense_index=-1)
+    sl = _slices(univ10_store, 4, 100, seed=61)
+    pbs = [wg.PreparedBatch(s) for s in sl]
+    for pb in pbs:
+        pb.status[:] = -7
+    ta = dev.batch_submit(pbs[0])
+    tb = dev.batch_submit(pbs[1], v2c=True)  # completes ta first
+    assert (pbs[0].status == 0).all()
+    with pytest.raises(wg.WkError) as e:    # ta's lane is idle but ta not waited for
+        dev.batch_submit(pbs[2])
+    assert e.value.code == wg.WK_E_STATE
+    dev.batch_wait(ta)
+    tc = dev.batch_submit(pbs[2])           # ta's lane again; completes tb
+    sync = dev.batch(sl[3])                 # completes tc, then runs on tc's lane
+    dev.batch_wait(tb)
+    dev.batch_wait(tc)
+    for k in range(3):
+        assert (pbs[k].status == 0).all()
+        for (pats, nv), r in zip(sl[k], pbs[k].rows):
+            assert int(r) == univ10_oracle.query(pats, nv)[0].shape[0], pats
+    _check(univ10_oracle, sl[3], sync)
+    with pytest.raises(wg.WkError) as e:    # tc's arena was reused by the synchronous batch
+        dev.batch_fetch(tc, pbs[2])
+    assert e.value.code == wg.WK_E_STATE
+    _check(univ10_oracle, sl[1], dev.batch_fetch(tb, pbs[1]))  # tb's lane was not reused
+    td = dev.batch_submit(pbs[0], v2c=True)
+    dev.batch_wait(td)
+    _check(univ10_oracle, sl[0], dev.batch_fetch(td, pbs[0]))
     dev.close()
 
 

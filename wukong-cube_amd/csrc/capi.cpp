@@ -323,6 +323,7 @@ int finish_lane(wk_dev* dev) {
                               dev->launch);
   dev->batch_used = rc ? 0 : dev->launch.arena_end;
   dev->fetchable = rc == WK_OK;
+  dev->done_rc = rc;
   return rc;
 }
 
@@ -361,15 +362,15 @@ int ensure_alt_lane(wk_dev* dev, int lane) {
   return rc;
 }
 
-int drain_batches(wk_dev* dev) {
+int drain_batches(wk_dev* dev, int lanes_used) {
   int rc = WK_OK;
   for (int lane = 0; lane < kMaxLanes; lane++) {
     if (!lane_state(dev, lane).inflight) continue;
     const int r = on_lane(dev, lane, [&]() -> int { return finish_lane(dev); });
     if (!rc) rc = r;
   }
-  // the call that drains may reuse the arenas
-  for (int lane = 0; lane < kMaxLanes; lane++) lane_state(dev, lane).fetchable = false;
+  // the call that drains reuses these lanes' arenas
+  for (int lane = 0; lane < lanes_used && lane < kMaxLanes; lane++) lane_state(dev, lane).fetchable = false;
   return rc;
 }
 
@@ -531,7 +532,7 @@ int wk_dev_set_profile(wk_dev* dev, int32_t on) {
 
 int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
   if (!dev) return WK_E_INVALID_ARG;
-  if (int drc = drain_batches(dev)) return drc;
+  if (int drc = drain_batches(dev, 0)) return drc;
   if (probe_width != 1 && probe_width != 2 && probe_width != 4 && probe_width != 8) return WK_E_INVALID_ARG;
   if (rows_per_wave != 0 && rows_per_wave != 1 && rows_per_wave != 4 && rows_per_wave != 16 &&
       rows_per_wave != 64)
@@ -553,7 +554,7 @@ int wk_dev_set_tuning(wk_dev* dev, int32_t probe_width, int32_t rows_per_wave) {
 
 int wk_dev_sync(wk_dev* dev) {
   if (!dev) return WK_E_INVALID_ARG;
-  if (int drc = drain_batches(dev)) return drc;
+  if (int drc = drain_batches(dev, 0)) return drc;
   WK_HIP(hipStreamSynchronize(dev->copy_stream));
   WK_HIP(hipStreamSynchronize(dev->stream));
   return WK_OK;
@@ -561,7 +562,7 @@ int wk_dev_sync(wk_dev* dev) {
 
 int wk_dev_load_all(wk_dev* dev) {
   if (!dev) return WK_E_INVALID_ARG;
-  if (int drc = drain_batches(dev)) return drc;
+  if (int drc = drain_batches(dev, 0)) return drc;
   std::vector<int> none;
   for (size_t i = 0; i < dev->seg.size(); i++) {
     std::vector<int> one{int(i)};
@@ -575,7 +576,7 @@ int wk_dev_load_all(wk_dev* dev) {
 
 int wk_seg_ensure(wk_dev* dev, uint32_t index, uint32_t pid, int32_t dir) {
   if (!dev || (dir != WK_IN && dir != WK_OUT)) return WK_E_INVALID_ARG;
-  if (int drc = drain_batches(dev)) return drc;
+  if (int drc = drain_batches(dev, 0)) return drc;
   const int s = dev->st->seg_index(index, pid, uint32_t(dir));
   if (s < 0) return WK_E_INVALID_ARG;
   std::vector<int> one{s};
@@ -876,7 +877,6 @@ int wk_batch_run(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_graph
   uint64_t end = 0;
   rc = run_batch(dev, dev->plan_cache, 0, nq, 0, use_graph != 0, out->rows, out->offset, out->status, &end);
   dev->batch_used = rc ? 0 : end;
-  dev->ticket = 0;
   return rc;
 }
 
@@ -890,7 +890,10 @@ int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
   int lane = -1;
   for (int i = 0; i < nl && lane < 0; i++) {
     const int c = (dev->next_lane + i) % nl;
-    if (!lane_state(dev, c).inflight) lane = c;
+    const LaneState& l = lane_state(dev, c);
+    // a lane whose last batch finished (completed by another call) but was
+    // not waited for yet keeps that ticket until its wait
+    if (!l.inflight && l.awaited) lane = c;
   }
   if (lane < 0) return WK_E_STATE;
   if (lane > 0) {
@@ -911,6 +914,8 @@ int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
   const int rc = on_lane(dev, lane, [&]() -> int {
     compile_batch(dev, qs, nq, out);
     dev->ticket = t;
+    dev->awaited = false;
+    dev->done_rc = WK_OK;
     dev->fetchable = false;
     dev->batch_used = 0;
     dev->b_out = *out;
@@ -920,6 +925,7 @@ int wk_batch_submit(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
                               dev->launch);
     if (r) {
       dev->ticket = 0;
+      dev->awaited = true;
       return r;
     }
     // a batch that already completed (no steps, or split for lack of
@@ -938,7 +944,8 @@ int wk_batch_wait(wk_dev* dev, uint32_t ticket) {
   for (int lane = 0; lane < kMaxLanes; lane++) {
     LaneState& l = lane_state(dev, lane);
     if (l.ticket != ticket) continue;
-    if (!l.inflight) return WK_OK;  // already finished
+    l.awaited = true;
+    if (!l.inflight) return l.done_rc;  // already finished, by a wait or by another call
     return on_lane(dev, lane, [&]() -> int { return finish_lane(dev); });
   }
   return WK_E_STATE;
@@ -1018,7 +1025,7 @@ int wk_batch_traffic(wk_dev* dev, int32_t enable, double* alg_bytes, double* edg
 
 int wk_batch_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems, uint64_t* used_elems) {
   if (!dev) return WK_E_INVALID_ARG;
-  if (int drc = drain_batches(dev)) return drc;
+  if (int drc = drain_batches(dev, 0)) return drc;
   if (used_elems) *used_elems = dev->batch_used;
   if (!dst) return WK_OK;
   if (cap_elems < dev->batch_used) return WK_E_INVALID_ARG;

@@ -143,10 +143,14 @@ struct LaneState {
 
   // A batch submitted on this lane (wk_batch_submit) whose GPU work may
   // still run: its launch state and the caller's result arrays, which
-  // finish_batch fills.  `ticket` stays the ticket of the batch whose
-  // results the arena holds until the lane's next batch.
+  // finish_batch fills.  `ticket` stays the lane's last submitted ticket
+  // until its next submit, whatever else runs on the lane in between, so
+  // that its wait still answers with the status it finished with; the lane
+  // takes no new submit before that wait.
   bool inflight = false;
   bool fetchable = false;  // the arena still holds that batch's tables
+  bool awaited = true;     // the caller has waited for the lane's last ticket
+  int done_rc = 0;         // what the lane's last batch finished with
   uint32_t ticket = 0;
   uint32_t b_nq = 0;
   bool b_graph = false;
@@ -297,8 +301,9 @@ int run_query_to(wk_dev* dev, const Plan& plan, uint32_t* dst, uint64_t cap, uin
 int ensure_alt_lane(wk_dev* dev, int lane = 1);
 // Completes every batch submitted with wk_batch_submit and not yet waited
 // for (their result arrays are filled); every entry point that uses a lane
-// calls it first.
-int drain_batches(wk_dev* dev);
+// calls it first.  The arenas of lanes [0, lanes_used), which the caller is
+// about to reuse, can no longer be fetched by ticket.
+int drain_batches(wk_dev* dev, int lanes_used = 1);
 
 // Batched execution of queries [q0, q1) of `plans`, in two halves:
 // begin_batch fills and uploads the step descriptors, makes the segments

@@ -344,7 +344,7 @@ using namespace wk;
 extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const double* arrival_s, uint32_t nq,
                                const wk_serve_opts* opts, wk_serve_result* out) {
   if (!dev || (!queries && nq) || (!arrival_s && nq) || !opts || !out) return WK_E_INVALID_ARG;
-  if (int drc = drain_batches(dev)) return drc;
+  if (int drc = drain_batches(dev, 2)) return drc;  // the main lane and the heavy-query lane
   for (uint32_t i = 1; i < nq; i++)
     if (arrival_s[i] < arrival_s[i - 1]) return WK_E_INVALID_ARG;
   std::vector<Plan> plans(nq);
