@@ -1173,6 +1173,66 @@ __device__ __forceinline__ uint64_t refine_row(const StepArgs& a, uint64_t g0, u
   return g0;
 }
 
+// What a mixed tile stages in LDS of each of its queries (up to kQStage):
+// first input row, first output row, the checksum share, and the descriptor
+// and segment fields a row's copy needs, so that a row reads no descriptor or
+// segment from memory.
+struct QStage {
+  uint64_t r0[kQStage];
+  uint64_t p0[kQStage];
+  unsigned long long ck[kQStage];
+  uint64_t out_off[kQStage];  // element offset of its table
+  uint64_t in_off[kQStage];
+  int64_t voff[kQStage];      // the expanded segment's edge addressing (DevSeg)
+  uint32_t vb_base[kQStage];
+  uint32_t meta[kQStage];     // in_cols | out_cols << 8 | expand << 16 | final << 17 | no room << 18
+};
+
+__device__ __forceinline__ void stage_query(const StepArgs& a, QStage& qs, uint32_t i, uint32_t q, uint64_t R) {
+  const StepDesc& d = a.desc[q];
+  const uint64_t r0 = a.row_start[q];
+  const bool expand = d.op == OP_EXPAND;
+  qs.r0[i] = r0;
+  qs.ck[i] = 0;
+  qs.out_off[i] = d.out_off;
+  qs.in_off[i] = d.in_off;
+  qs.meta[i] = d.in_cols | (d.out_cols << 8) | (expand ? 1u << 16 : 0u) | (d.final_step ? 1u << 17 : 0u) |
+               (d.out_off == kNoSpace ? 1u << 18 : 0u);
+  const DevSeg& s = a.segs[expand ? d.seg : 0];
+  qs.voff[i] = s.voff;
+  qs.vb_base[i] = s.vb_base;
+  qs.p0[i] = pfx_at(a, r0 < R ? r0 : R);
+}
+
+// emit_row for query i of the tile's staged queries.
+template <bool CK>
+__device__ __forceinline__ uint64_t emit_row_staged(const StepArgs& a, const QStage& qs, uint32_t i, uint64_t g,
+                                                    uint64_t j, uint64_t pg, uint32_t eo) {
+  const uint32_t m = qs.meta[i];
+  if (m & (1u << 18)) return 0;  // final table dropped: no room in the arena
+  const uint32_t ic = m & 0xffu, oc = (m >> 8) & 0xffu;
+  const bool expand = (m >> 16) & 1u, fin = (m >> 17) & 1u;
+  const uint64_t orow = j - qs.p0[i] - a.win_lo;
+  uint32_t* dst = (fin ? a.arena : a.tout) + qs.out_off[i] + orow * oc;
+  const uint32_t* src = a.tin + qs.in_off[i] + (g - qs.r0[i]) * ic;
+  const bool ck = CK && a.ck && fin;
+  uint64_t h = row_hash_init(oc);
+  for (uint32_t c = 0; c < ic; ++c) {
+    const uint32_t x = src[c];
+    dst[c] = x;
+    if (ck) h = row_hash_step(h, x, c);
+  }
+  if (expand) {
+    DevSeg sg;  // the fields edge_at reads
+    sg.voff = qs.voff[i];
+    sg.vb_base = qs.vb_base[i];
+    const uint32_t x = edge_at(a, sg, uint64_t(eo) + (j - pg));
+    dst[ic] = x;
+    if (ck) h = row_hash_step(h, x, ic);
+  }
+  return ck ? h : 0;
+}
+
 // The rows of a tile that spans several queries: each output row finds its
 // source row (in the staged rows, or bracketed by the staged samples and
 // finished in the global prefix) and its query (in LDS when the tile's
@@ -1183,11 +1243,13 @@ template <bool CK, bool STAGED>
 __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uint64_t total, uint64_t lo,
                                              uint64_t hi, uint64_t S, uint32_t ns, uint32_t qlo, uint32_t qhi,
                                              bool qlds, const int32_t* s_rel, const uint32_t* s_eoff,
-                                             const uint64_t* s_qr0, const uint64_t* s_qp0,
-                                             unsigned long long* s_ck) {
+                                             QStage& qs) {
+  // this thread's index, opaque to the compiler (fewer registers held)
+  uint32_t tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
 #pragma unroll
   for (int k = 0; k < kOutTile / kBlock; ++k) {
-    const uint64_t j = j0 + uint64_t(k) * kBlock + threadIdx.x;
+    const uint64_t j = j0 + uint64_t(k) * kBlock + tid;
     if (j >= total) continue;
     const int32_t jl = int32_t(j - j0);
     uint32_t l = 0, h = ns;
@@ -1212,12 +1274,12 @@ __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uin
       uint32_t ql = 0, qh = qhi - qlo;
       while (qh - ql > 1) {
         const uint32_t m = (ql + qh) >> 1;
-        if (s_qr0[m] <= g) ql = m;
+        if (qs.r0[m] <= g) ql = m;
         else qh = m;
       }
       // checksum shares gather in LDS, one global add per query and tile
-      const uint64_t rh = emit_row<CK>(a, qlo + ql, s_qr0[ql], s_qp0[ql], g, j, pg, eo);
-      if (CK && rh) atomicAdd(&s_ck[ql], (unsigned long long)rh);
+      const uint64_t rh = emit_row_staged<CK>(a, qs, ql, g, j, pg, eo);
+      if (CK && rh) atomicAdd(&qs.ck[ql], (unsigned long long)rh);
     } else {
       const uint32_t q = query_in(a.row_start, qlo, qhi, g);
       const uint64_t rh = emit_row<CK>(a, q, g, j, pg, eo);
@@ -1226,7 +1288,7 @@ __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uin
   }
   if (CK && a.ck && qlds) {
     __syncthreads();
-    if (threadIdx.x < qhi - qlo && s_ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], s_ck[threadIdx.x]);
+    if (threadIdx.x < qhi - qlo && qs.ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], qs.ck[threadIdx.x]);
   }
 }
 
@@ -1268,9 +1330,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
   // edges), clamped above
   __shared__ int32_t s_rel[kStage];
   __shared__ uint32_t s_eoff[kStage];
-  __shared__ uint64_t s_qr0[kQStage];  // first input row of each query of the tile
-  __shared__ uint64_t s_qp0[kQStage];  // and its first output row
-  __shared__ unsigned long long s_ck[kQStage];  // and its checksum share from this tile
+  __shared__ QStage s_qs;  // the queries of a tile that spans several
   const uint64_t R = step_rows(a);
   const uint64_t total = out_end(a);
   const uint64_t nt = out_tiles(a);
@@ -1345,12 +1405,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
     // the queries of a tile that spans several: first row and first output
     // row of each, staged in LDS when they fit
     const bool qlds = qhi - qlo > 1 && qhi - qlo <= uint32_t(kQStage);
-    if (qlds && tid < qhi - qlo) {
-      const uint64_t r0 = a.row_start[qlo + tid];
-      s_qr0[tid] = r0;
-      s_qp0[tid] = pfx_at(a, r0 < R ? r0 : R);
-      s_ck[tid] = 0;
-    }
+    if (qlds && tid < qhi - qlo) stage_query(a, s_qs, tid, qlo + tid, R);
     __syncthreads();
     if (staged && qhi - qlo == 1) {
       // one query owns the whole tile: its descriptor, row start and segment
@@ -1460,9 +1515,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
         if (tid == 0 && hsum) atomicAdd(&a.ck[q], (unsigned long long)hsum);
       }
     } else if (staged) {
-      gather_mixed<CK, true>(a, j0, total, lo, hi, 1, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qr0, s_qp0, s_ck);
+      gather_mixed<CK, true>(a, j0, total, lo, hi, 1, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qs);
     } else {
-      gather_mixed<CK, false>(a, j0, total, lo, hi, S, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qr0, s_qp0, s_ck);
+      gather_mixed<CK, false>(a, j0, total, lo, hi, S, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qs);
     }
     __syncthreads();
   }
