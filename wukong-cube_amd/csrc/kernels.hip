@@ -1173,11 +1173,33 @@ __device__ __forceinline__ uint64_t refine_row(const StepArgs& a, uint64_t g0, u
   return g0;
 }
 
+// Rank of this thread's flag among the set flags of its kBlock-thread block,
+// in thread order, and their number.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_flag_rank(bool f, uint32_t& total) {
+  __shared__ uint32_t s_wc[kBlock / kWave];
+  const uint64_t m = __ballot(f);
+  const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint32_t below = uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)));
+  if (lane == 0) s_wc[w] = uint32_t(__popcll(m));
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / kWave; ++i) {
+    const uint32_t c = s_wc[i];
+    off += uint32_t(i) < w ? c : 0u;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return off + below;
+}
+
 // What a mixed tile stages in LDS of each of its queries (up to kQStage):
 // first input row, first output row, the checksum share, and the descriptor
 // and segment fields a row's copy needs, so that a row reads no descriptor or
 // segment from memory.
 struct QStage {
+  uint32_t qid[kQStage];       // the query
   uint64_t r0[kQStage];
   uint64_t p0[kQStage];
   unsigned long long ck[kQStage];
@@ -1192,6 +1214,7 @@ __device__ __forceinline__ void stage_query(const StepArgs& a, QStage& qs, uint3
   const StepDesc& d = a.desc[q];
   const uint64_t r0 = a.row_start[q];
   const bool expand = d.op == OP_EXPAND;
+  qs.qid[i] = q;
   qs.r0[i] = r0;
   qs.ck[i] = 0;
   qs.out_off[i] = d.out_off;
@@ -1242,8 +1265,9 @@ __device__ __forceinline__ uint64_t emit_row_staged(const StepArgs& a, const QSt
 template <bool CK, bool STAGED>
 __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uint64_t total, uint64_t lo,
                                              uint64_t hi, uint64_t S, uint32_t ns, uint32_t qlo, uint32_t qhi,
-                                             bool qlds, const int32_t* s_rel, const uint32_t* s_eoff,
+                                             uint32_t nqs, const int32_t* s_rel, const uint32_t* s_eoff,
                                              QStage& qs) {
+  const bool qlds = nqs > 0;
   // this thread's index, opaque to the compiler (fewer registers held)
   uint32_t tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
@@ -1271,7 +1295,7 @@ __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uin
       eo = a.eoff[g];
     }
     if (qlds) {
-      uint32_t ql = 0, qh = qhi - qlo;
+      uint32_t ql = 0, qh = nqs;
       while (qh - ql > 1) {
         const uint32_t m = (ql + qh) >> 1;
         if (qs.r0[m] <= g) ql = m;
@@ -1288,7 +1312,7 @@ __device__ __forceinline__ void gather_mixed(const StepArgs& a, uint64_t j0, uin
   }
   if (CK && a.ck && qlds) {
     __syncthreads();
-    if (threadIdx.x < qhi - qlo && qs.ck[threadIdx.x]) atomicAdd(&a.ck[qlo + threadIdx.x], qs.ck[threadIdx.x]);
+    if (threadIdx.x < nqs && qs.ck[threadIdx.x]) atomicAdd(&a.ck[qs.qid[threadIdx.x]], qs.ck[threadIdx.x]);
   }
 }
 
@@ -1402,10 +1426,30 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
     uint32_t qlo, qlast;
     block_find_queries<kBlock>(a.row_start, a.nq, lo, hi - 1, qlo, qlast);
     const uint32_t qhi = qlast + 1;
-    // the queries of a tile that spans several: first row and first output
-    // row of each, staged in LDS when they fit
-    const bool qlds = qhi - qlo > 1 && qhi - qlo <= uint32_t(kQStage);
-    if (qlds && tid < qhi - qlo) stage_query(a, s_qs, tid, qlo + tid, R);
+    // the queries of a tile that spans several, staged in LDS when they fit:
+    // all of them, or those with rows when more lie between (queries that
+    // skip this step have none; not in the checksum variant, whose registers
+    // it would cost a wave)
+    uint32_t nqs = 0;
+    if (qhi - qlo > 1) {
+      const bool all = CK || qhi - qlo <= uint32_t(kQStage);  // then one pass (kQStage = kBlock)
+      for (uint32_t q0 = qlo; q0 < qhi && nqs <= uint32_t(kQStage); q0 += kBlock) {
+        const uint32_t q = q0 + tid;
+        bool has;
+        uint32_t at, cnt;
+        if (all) {
+          has = q < qhi;
+          at = tid;
+          cnt = qhi - qlo;
+        } else {
+          has = q < qhi && a.row_start[q + 1] > a.row_start[q];
+          at = nqs + block_flag_rank(has, cnt);
+        }
+        if (has && at < uint32_t(kQStage)) stage_query(a, s_qs, at, q, R);
+        nqs += cnt;
+      }
+      if (nqs > uint32_t(kQStage)) nqs = 0;  // too many: rows find their queries in memory
+    }
     __syncthreads();
     if (staged && qhi - qlo == 1) {
       // one query owns the whole tile: its descriptor, row start and segment
@@ -1515,9 +1559,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
         if (tid == 0 && hsum) atomicAdd(&a.ck[q], (unsigned long long)hsum);
       }
     } else if (staged) {
-      gather_mixed<CK, true>(a, j0, total, lo, hi, 1, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qs);
+      gather_mixed<CK, true>(a, j0, total, lo, hi, 1, ns, qlo, qhi, nqs, s_rel, s_eoff, s_qs);
     } else {
-      gather_mixed<CK, false>(a, j0, total, lo, hi, S, ns, qlo, qhi, qlds, s_rel, s_eoff, s_qs);
+      gather_mixed<CK, false>(a, j0, total, lo, hi, S, ns, qlo, qhi, nqs, s_rel, s_eoff, s_qs);
     }
     __syncthreads();
   }
