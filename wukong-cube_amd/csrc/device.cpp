@@ -578,11 +578,8 @@ int probe_scan(wk_dev* dev, const StepDesc& sd0, const uint32_t* tin, uint64_t R
   // one 4-B word
   timed_end(dev, ev, "k_probe", (sd.op == OP_FILTER_TYPE ? 20 : direct ? 24 : 32) * Rd);
   timed_begin(dev, &ev);
-  launch_scan_tiles(a, dev->stream);
-  timed_end(dev, ev, "k_scan_tiles", 16 * ntiles);
-  timed_begin(dev, &ev);
-  launch_scan_queries(a, dev->stream);
-  timed_end(dev, ev, "k_scan_queries", 96);
+  launch_scan_small(a, dev->stream);  // tile scan and query scan of the one query
+  timed_end(dev, ev, "k_scan_small", 16 * ntiles + 96);
   if (hip_failed(hipMemcpyAsync(dev->h_state, dev->d_state, sizeof(StepState), hipMemcpyDeviceToHost, dev->stream)) ||
       hip_failed(hipStreamSynchronize(dev->stream)))
     return WK_E_HIP;

@@ -806,13 +806,10 @@ __device__ __forceinline__ uint32_t step_status(const StepArgs& a, uint64_t carr
   return carry_t > a.tcap ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
-  if (a.st->status) {
-    // an overflowed step skips everything after it: pass the status on so
-    // the next step's kernels also return before reading its (unwritten) rows
-    if (threadIdx.x == 0 && a.next_st) a.next_st->status = a.st->status;
-    return;
-  }
+// The query scan of one step by one 1,024-thread block: every query's output
+// rows and offsets, the next step's row space and descriptors, the step's
+// status.  Every thread of the block must call it.
+__device__ __forceinline__ void scan_queries_block(const StepArgs& a) {
   const uint64_t R = step_rows(a);
   // each thread owns a contiguous run of queries: three block scans in all
   const uint32_t per = (a.nq + 1023) / 1024;
@@ -947,6 +944,32 @@ __global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
       }
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_queries(StepArgs a) {
+  if (a.st->status) {
+    // an overflowed step skips everything after it: pass the status on so
+    // the next step's kernels also return before reading its (unwritten) rows
+    if (threadIdx.x == 0 && a.next_st) a.next_st->status = a.st->status;
+    return;
+  }
+  scan_queries_block(a);
+}
+
+// The tile scan and the query scan of a step in one 1,024-thread block (steps
+// of at most kBlock queries: every single query, small serving iterations):
+// the query scan reads the tile prefixes the same block wrote before the
+// barrier between them.
+__global__ __launch_bounds__(1024) void k_scan_small(StepArgs a) {
+  if (a.st->status) {
+    // an overflowed step skips everything after it: pass the status on so
+    // the next step's kernels also return before reading its (unwritten) rows
+    if (threadIdx.x == 0 && a.next_st) a.next_st->status = a.st->status;
+    return;
+  }
+  scan_tiles_block<1024>(a);
+  __syncthreads();
+  scan_queries_block(a);
 }
 
 // Multi-block form of k_scan_queries for large batches.  One block scanning
@@ -1975,6 +1998,10 @@ void launch_scan_queries(const StepArgs& a, hipStream_t s) {
   WK_LAUNCH(k_scan_queries, dim3(1), dim3(1024), 0, s, a);
 }
 
+void launch_scan_small(const StepArgs& a, hipStream_t s) {
+  WK_LAUNCH(k_scan_small, dim3(1), dim3(1024), 0, s, a);
+}
+
 void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
   const uint32_t nqb = (a.nq + kBlock - 1) / kBlock;
   // blocks past nqb do k_tile_rows' work
@@ -1984,20 +2011,17 @@ void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
 }
 
 void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s) {
-  launch_scan_tiles(a, s);
   if (a.nq > uint32_t(kBlock) && a.qagg) {
+    launch_scan_tiles(a, s);
     launch_scan_queries_mb(a, s);  // output tiles are marked by its first launch
     launch_gather_only(a, rows_per_wave, 0, s);
   } else {
-    launch_scan_queries(a, s);
+    launch_scan_small(a, s);
     launch_gather(a, rows_per_wave, 0, s);
   }
 }
 
-void launch_scan(const StepArgs& a, hipStream_t s) {
-  launch_scan_tiles(a, s);
-  launch_scan_queries(a, s);
-}
+void launch_scan(const StepArgs& a, hipStream_t s) { launch_scan_small(a, s); }
 
 void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s) {
   const uint64_t tiles = (out_hint + kOutTile - 1) / kOutTile;

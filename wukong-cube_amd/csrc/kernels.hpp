@@ -143,15 +143,17 @@ constexpr int kOutTile = 4 * kBlock;            // output rows per gather tile (
 // re-read the device sizes and stride).  `rows_hint`/`out_hint` = 0 means
 // unknown on the host: a fixed persistent grid is used.
 void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s);
-// scan = scan_tiles + scan_queries
-// scan_tiles, the query scan (multi-block for more than kBlock queries, which
-// also marks output tiles) and the gather of one step with no host size
-// hints: the batched and served paths.
+// scan = scan_tiles + scan_queries, in one launch (launch_scan_small) for
+// steps of at most kBlock queries.
+// The scans (multi-block query scan for more than kBlock queries, which also
+// marks output tiles) and the gather of one step with no host size hints: the
+// batched and served paths.
 void launch_expand_tail(const StepArgs& a, int rows_per_wave, hipStream_t s);
 void launch_scan_queries_mb(const StepArgs& a, hipStream_t s);
 void launch_scan(const StepArgs& a, hipStream_t s);
 void launch_scan_tiles(const StepArgs& a, hipStream_t s);
 void launch_scan_queries(const StepArgs& a, hipStream_t s);
+void launch_scan_small(const StepArgs& a, hipStream_t s);
 // gather = tile_rows (output-balanced mapping only) + the gather kernel
 void launch_gather(const StepArgs& a, int rows_per_wave, uint64_t out_hint, hipStream_t s);
 void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s);
