@@ -24,8 +24,15 @@ void launch_batch_steps(wk_dev* dev, uint32_t k0, uint32_t k1, uint32_t S, uint3
     StepArgs a = make_args(dev, k, nq, tin, tout, dev->tcap, k + 1 < S);
     a.qres = dev->d_qres;
     hipEvent_t ev = nullptr;
+    int w = dev->opts.probe_width;
+    if (nq == 1) {
+      // a key read from a direct index is one 8-B word, so lanes sharing a
+      // key would only repeat it: one lane per key (as the per-query path)
+      const StepDesc& d = dev->h_desc[size_t(k) * dev->nq_cap];
+      if (d.seg >= 0 && dev->h_segs[d.seg].dense && d.key_mode != KEY_TYPE_INDEX) w = 1;
+    }
     if (timed) timed_begin(dev, &ev);
-    launch_probe(a, dev->opts.probe_width, 0, dev->stream);
+    launch_probe(a, w, 0, dev->stream);
     if (timed) timed_end(dev, ev, "batch_probe", 0);
     if (timed) timed_begin(dev, &ev);
     launch_expand_tail(a, dev->opts.rows_per_wave, dev->stream);
@@ -277,6 +284,20 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
   if (!rc) rc = finish_batch(dev, plans, q0, q1, use_graph, rows, offsets, status, bl);
   *arena_end = bl.arena_end;
   return rc;
+}
+
+int run_one(wk_dev* dev, const Plan& plan, uint64_t* rows) {
+  *rows = 0;
+  if (dev->opts.profile) return run_query_to(dev, plan, dev->arena, dev->arena_cap, rows);
+  if (plan.status || plan.steps.empty()) return plan.status;
+  dev->one_plan.resize(1);
+  dev->one_plan[0] = plan;
+  uint64_t offset = 0, end = 0;
+  int32_t status = WK_OK;
+  // direct launches: for one query they measured faster than a graph replay
+  const int rc = run_batch(dev, dev->one_plan, 0, 1, 0, false, rows, &offset, &status, &end);
+  if (rc) return rc;
+  return status;  // a table that fits starts at offset 0: the query's arena base
 }
 
 }  // namespace wk

@@ -843,7 +843,7 @@ int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols,
   uint64_t r = 0;
   // the result goes to the result arena, in place of the last batch's
   dev->batch_used = 0;
-  int rc = run_query_to(dev, plan, dev->arena, dev->arena_cap, &r);
+  int rc = run_one(dev, plan, &r);
   if (rc) return rc;
   dev->last_ptr = dev->arena;
   dev->last_rows = r;

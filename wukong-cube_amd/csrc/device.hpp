@@ -138,6 +138,8 @@ struct LaneState {
 
   // compiled plans of the lane's last batch (storage reused by the next)
   std::vector<Plan> plan_cache;
+  // the plan of a query run by itself (run_one)
+  std::vector<Plan> one_plan;
   // elements of the arena holding the last batch's results
   uint64_t batch_used = 0;
 
@@ -321,6 +323,13 @@ int run_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t
 // Queries [q0, q1) as two batches, one after the other.
 int split_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32_t q1, uint64_t arena_base,
                 bool use_graph, uint64_t* rows, uint64_t* offsets, int32_t* status, uint64_t* arena_end);
+// One query by itself, its result at the start of the lane's result arena
+// and its rows in *rows; returns the query's status (or WK_E_HIP).  Its steps
+// run from device-side sizes with one host round trip: the batch path with
+// one query, whose overflowing step falls back to the chunked executor.  With
+// per-kernel profiling on, the step-by-step path runs instead, whose launches
+// carry the host's row counts and algorithmic bytes.
+int run_one(wk_dev* dev, const Plan& plan, uint64_t* rows);
 // One query of a batch by itself (per-query path, chunked as needed), its
 // result into the arena at arena_base.
 int run_alone(wk_dev* dev, const Plan& plan, uint64_t arena_base, uint64_t* rows, uint64_t* offset, int32_t* status,

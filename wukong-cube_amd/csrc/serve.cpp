@@ -412,11 +412,11 @@ extern "C" int wk_serve_replay(wk_dev* dev, const wk_query* queries, const doubl
     int hr = heavy.ready();
     if (hr) return hr;
   }
-  // one query by itself, pattern by pattern with a host sync per pattern,
-  // its result into the (then unused) result arena
+  // one query by itself (its lane has nothing else in flight), its result
+  // into the (then unused) result arena
   auto run_alone = [&](uint32_t q) -> int {
     uint64_t rows = 0;
-    const int st = run_query_to(dev, plans[q], dev->arena, dev->arena_cap, &rows);
+    const int st = run_one(dev, plans[q], &rows);
     if (st == WK_E_HIP) return st;
     uint64_t ck = 0;
     if (st == WK_OK && rows) {
