@@ -13,7 +13,7 @@ Default workload: the university-schema graph at 2,560 universities (seed 1,
     The native autotuner first picks the probe width and the gather mapping.
     The steps' algorithmic bytes, counted from the device's own row counts in
     a separate pass, give the batch's roofline;
-  * the same stream per query (one host sync per pattern) for comparison;
+  * the same stream one query at a time (wk_query_run) for comparison;
   * open-loop Poisson serving replay, continuous batching vs per-query, p50/p99;
   * the C oracle on the host cores as the CPU baseline: one worker per thread
     pulling queries from a shared queue over the timed slices, and the single

@@ -295,10 +295,14 @@ int wk_expand_c2k(wk_dev* dev, const wk_table* in, uint32_t vid, uint32_t pid, i
 
 /* ------------------------------------------------------------------ query */
 
-/* Run one query pattern by pattern (per-query GPU execution, one host sync
- * per pattern as in the reference).  On success the history table stays on
- * the device as the handle's last result: shape and var->column map are
- * returned here, rows are fetched with wk_result_download. */
+/* Run one query (per-query GPU execution; the reference's loop of one
+ * execute_one_pattern per pattern).  Its pattern steps run from sizes kept on
+ * the device, with one host round trip for the query; a step whose tables
+ * exceed the capacity runs in chunks, depth-first, with a host sync per
+ * chunk.  With opts.profile on, every step syncs, so that each launch is
+ * timed with its row counts.  On success the history table stays on the
+ * device as the handle's last result: shape and var->column map are returned
+ * here, rows are fetched with wk_result_download. */
 int wk_query_run(wk_dev* dev, const wk_query* q, uint64_t* rows, uint32_t* cols, int32_t* v2c);
 int wk_result_download(wk_dev* dev, uint32_t* dst, uint64_t cap_elems);
 

@@ -7,7 +7,7 @@ not been built: there is no Python fallback.
 Mirrors the reference's GPU engine flow: a `Store` is the host gstore
 (SegmentRDFGraph), a `Device` owns the HBM gstore cache and history tables
 (GPUMem + GPUCache + GPUEngine), `Device.query` runs a planned pattern list
-pattern by pattern (GPUAgent::execute_sparql_query).
+as one query (GPUAgent::execute_sparql_query).
 """
 import ctypes as C
 import os
