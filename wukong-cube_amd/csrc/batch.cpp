@@ -32,7 +32,7 @@ void launch_batch_steps(wk_dev* dev, uint32_t k0, uint32_t k1, uint32_t S, uint3
       if (d.seg >= 0 && dev->h_segs[d.seg].dense && d.key_mode != KEY_TYPE_INDEX) w = 1;
     }
     if (timed) timed_begin(dev, &ev);
-    launch_probe(a, w, 0, dev->stream);
+    launch_probe(a, w, 0, dev->stream, k == 0);  // step 0 reads one row per query
     if (timed) timed_end(dev, ev, "batch_probe", 0);
     if (timed) timed_begin(dev, &ev);
     launch_expand_tail(a, dev->opts.rows_per_wave, dev->stream);

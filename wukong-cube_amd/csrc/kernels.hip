@@ -628,22 +628,23 @@ __device__ __forceinline__ void probe_tile_end(const StepArgs& a, uint64_t t, ui
 // One workgroup of NT threads per tile of kTile rows, in kTile / (NT / W)
 // passes.  A tile inside one query reads the descriptor and segment once,
 // into scalar registers; a tile spanning several queries stages their row
-// starts in LDS and each row finds its query there.  NT = 1024 (one row per
-// lane at W = 1) serves batches, whose tiles mix queries; NT = 256 serves one
-// query, whose descriptor and segment each workgroup reads once for all its
-// tiles.  (Capping the registers for one more wave per SIMD made the large
-// query's probe 30% slower: fewer loads stay in flight per wave.)
+// starts, descriptors and input offsets in LDS and each row finds its query
+// there.  NT = 512 (two rows per lane at W = 1, two workgroups per CU) serves
+// batches; NT = 1024 a batched step of one row per query, whose tiles span
+// as many queries as rows; NT = 256 serves one query, whose descriptor and
+// segment each workgroup reads once for all its tiles.  (Capping the
+// registers for one more wave per SIMD made the large query's probe 30%
+// slower: fewer loads stay in flight per wave.)
 template <int W, int NT>
 __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
   static_assert(kTile % NT == 0 && NT % W == 0, "probe tile shape");
   if (a.st->status) return;
-  // the queries of a tile that spans several, staged by the batch probe
-  // only (a one-query launch has no such tile)
-  constexpr int QS = NT == 1024 ? NT : 1;
+  // the queries of a tile that spans several (up to a tile's rows), staged by
+  // the batch probes only (a one-query launch has no such tile)
+  constexpr int QS = NT >= 512 ? kTile : 1;
   __shared__ uint64_t s_rs[QS];
   __shared__ ProbeDesc s_pd[QS];
   __shared__ uint64_t s_in[QS];
-  __shared__ DevSeg s_sg[QS];
   // the tile's counts, scanned at its end; with several rows per thread the
   // prefixes go through s_pfx, so that their stores are coalesced
   constexpr int kPer = kTile / NT;
@@ -654,8 +655,8 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
   constexpr int G = NT / W;
   constexpr int kPasses = kTile / G;
   const int grp = threadIdx.x / W, gl = threadIdx.x % W;
-  if constexpr (NT != 1024) {
-    // one query (launch_probe uses NT = 1024 for every launch of several):
+  if constexpr (NT == 256) {
+    // one query (launch_probe uses NT = 256 for one-query launches only):
     // its descriptor, segment and row base are read from memory once per
     // workgroup, into LDS, and from there for each tile
     __shared__ ProbeDesc s_d1[1];
@@ -694,16 +695,16 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
       } else {
         const uint32_t nqt = q1 - q0 + 1;
         const bool staged = QS > 1 && nqt <= uint32_t(QS);
-        if (staged && threadIdx.x < nqt) {
-          // the tile's queries: row start, probe descriptor, input offset and
-          // segment, so that a row's chain starts from LDS instead of three
-          // dependent global loads
-          const uint32_t q = q0 + threadIdx.x;
-          s_rs[threadIdx.x] = a.row_start[q];
-          const ProbeDesc d = read_probe_desc(a.desc + q);
-          s_pd[threadIdx.x] = d;
-          s_in[threadIdx.x] = a.desc[q].in_off;
-          if (d.op != OP_SKIP && d.seg >= 0) s_sg[threadIdx.x] = a.segs[d.seg];
+        if (staged) {
+          // the tile's queries: row start, probe descriptor and input offset,
+          // so that a row's chain starts from LDS instead of dependent global
+          // loads (its segment, shared by many queries, is read from cache)
+          for (uint32_t i = threadIdx.x; i < nqt; i += NT) {
+            const uint32_t q = q0 + i;
+            s_rs[i] = a.row_start[q];
+            s_pd[i] = read_probe_desc(a.desc + q);
+            s_in[i] = a.desc[q].in_off;
+          }
         }
         __syncthreads();
         for (int pass = 0; pass < kPasses; ++pass) {
@@ -722,7 +723,7 @@ __global__ __launch_bounds__(NT) void k_probe(StepArgs a) {
               if (d.op == OP_FILTER_TYPE)
                 c = type_bit(a, d, a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols);
               else if (d.op != OP_SKIP && d.seg >= 0)
-                probe_row<W>(a, d, s_sg[l], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
+                probe_row<W>(a, d, a.segs[d.seg], a.tin + s_in[l] + (g - s_rs[l]) * d.in_cols, c, eo);
             } else {
               const uint32_t q = query_in(a.row_start, q0, q1 + 1, g);
               const ProbeDesc d = read_probe_desc(a.desc + q);
@@ -1909,14 +1910,16 @@ inline unsigned grid_for(uint64_t units, unsigned cap) {
 // rounds of workgroups that find no tile: the batch probe's 1,024-thread
 // workgroup holds 120 KB of LDS, one per CU, so 2,048 of them took 8 rounds.
 struct LaunchGrids {
-  unsigned probe[4];       // k_probe<W, 1024>, W = 1, 2, 4, 8
+  unsigned probe[4];       // k_probe<W, 512>, W = 1, 2, 4, 8
+  unsigned probe1k[4];     // k_probe<W, 1024>, steps of one row per query
   unsigned probe1q[4];     // k_probe<W, 256>, the one-query probe
   unsigned tile_rows;
   unsigned gather[2];      // k_gather<false>, k_gather<true>
   unsigned gather_rows[4]; // k_gather_rows<1, 4, 16, 64>
   unsigned query_sums;
 };
-LaunchGrids g_grids = {{kPersistentGrid, kPersistentGrid, kPersistentGrid, kPersistentGrid},
+LaunchGrids g_grids = {{2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid, 2 * kPersistentGrid},
+                       {kPersistentGrid, kPersistentGrid, kPersistentGrid, kPersistentGrid},
                        {4 * kPersistentGrid, 4 * kPersistentGrid, 4 * kPersistentGrid, 4 * kPersistentGrid},
                        kPersistentGrid,
                        {4 * kPersistentGrid, 4 * kPersistentGrid},
@@ -1945,10 +1948,14 @@ void init_launch_grids() {
     }
     auto rg = [cus](const void* k, int block, unsigned fallback) { return resident_grid(k, block, cus, fallback); };
     LaunchGrids g = g_grids;
-    g.probe[0] = rg(reinterpret_cast<const void*>(k_probe<1, 1024>), 1024, g.probe[0]);
-    g.probe[1] = rg(reinterpret_cast<const void*>(k_probe<2, 1024>), 1024, g.probe[1]);
-    g.probe[2] = rg(reinterpret_cast<const void*>(k_probe<4, 1024>), 1024, g.probe[2]);
-    g.probe[3] = rg(reinterpret_cast<const void*>(k_probe<8, 1024>), 1024, g.probe[3]);
+    g.probe[0] = rg(reinterpret_cast<const void*>(k_probe<1, 512>), 512, g.probe[0]);
+    g.probe1k[0] = rg(reinterpret_cast<const void*>(k_probe<1, 1024>), 1024, g.probe1k[0]);
+    g.probe[1] = rg(reinterpret_cast<const void*>(k_probe<2, 512>), 512, g.probe[1]);
+    g.probe1k[1] = rg(reinterpret_cast<const void*>(k_probe<2, 1024>), 1024, g.probe1k[1]);
+    g.probe[2] = rg(reinterpret_cast<const void*>(k_probe<4, 512>), 512, g.probe[2]);
+    g.probe1k[2] = rg(reinterpret_cast<const void*>(k_probe<4, 1024>), 1024, g.probe1k[2]);
+    g.probe[3] = rg(reinterpret_cast<const void*>(k_probe<8, 512>), 512, g.probe[3]);
+    g.probe1k[3] = rg(reinterpret_cast<const void*>(k_probe<8, 1024>), 1024, g.probe1k[3]);
     g.probe1q[0] = rg(reinterpret_cast<const void*>(k_probe<1, 256>), 256, g.probe1q[0]);
     g.probe1q[1] = rg(reinterpret_cast<const void*>(k_probe<2, 256>), 256, g.probe1q[1]);
     g.probe1q[2] = rg(reinterpret_cast<const void*>(k_probe<4, 256>), 256, g.probe1q[2]);
@@ -1965,27 +1972,33 @@ void init_launch_grids() {
   });
 }
 
+// Workgroup size of a probe launch: 256 threads for one query; for several,
+// 512 (two rows per lane at W = 1, two workgroups per CU), or 1,024 for a
+// step of one row per query, whose tiles each span a tile's worth of queries.
 template <int W>
-void probe_launch(const StepArgs& a, unsigned grid, bool one_query, hipStream_t s) {
-  if (one_query)
+void probe_launch(const StepArgs& a, unsigned grid, int nt, hipStream_t s) {
+  if (nt == 256)
     WK_LAUNCH((k_probe<W, 256>), dim3(grid), dim3(256), 0, s, a);
-  else
+  else if (nt == 1024)
     WK_LAUNCH((k_probe<W, 1024>), dim3(grid), dim3(1024), 0, s, a);
+  else
+    WK_LAUNCH((k_probe<W, 512>), dim3(grid), dim3(512), 0, s, a);
 }
 
-void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s) {
+void launch_probe(const StepArgs& a, int w, uint64_t rows_hint, hipStream_t s, bool row_per_query) {
   const int wi = w == 8 ? 3 : w == 4 ? 2 : w == 2 ? 1 : 0;
+  const int nt = a.nq == 1 ? 256 : row_per_query ? 1024 : 512;
   // the workgroups resident at once stride over the tiles (fewer when the
   // host knows the step has fewer tiles)
-  const unsigned resident = a.nq == 1 ? g_grids.probe1q[wi] : g_grids.probe[wi];
+  const unsigned resident = nt == 256 ? g_grids.probe1q[wi] : nt == 1024 ? g_grids.probe1k[wi] : g_grids.probe[wi];
   const unsigned grid = rows_hint ? grid_for((rows_hint + kTile - 1) / kTile, resident) : resident;
   // w lanes share one key: each reads 8/w slots of the 128-B bucket, so a
   // wave's loads of one slot index cover 64/w buckets, each line coalesced
   switch (w) {
-    case 8: probe_launch<8>(a, grid, a.nq == 1, s); break;
-    case 4: probe_launch<4>(a, grid, a.nq == 1, s); break;
-    case 2: probe_launch<2>(a, grid, a.nq == 1, s); break;
-    default: probe_launch<1>(a, grid, a.nq == 1, s); break;
+    case 8: probe_launch<8>(a, grid, nt, s); break;
+    case 4: probe_launch<4>(a, grid, nt, s); break;
+    case 2: probe_launch<2>(a, grid, nt, s); break;
+    default: probe_launch<1>(a, grid, nt, s); break;
   }
 }
 

@@ -142,7 +142,8 @@ constexpr int kOutTile = 4 * kBlock;            // output rows per gather tile (
 // Launch wrappers (all on `stream`; grids sized from host hints, kernels
 // re-read the device sizes and stride).  `rows_hint`/`out_hint` = 0 means
 // unknown on the host: a fixed persistent grid is used.
-void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s);
+// row_per_query: a batched step whose input is one row per query (its first).
+void launch_probe(const StepArgs& a, int probe_width, uint64_t rows_hint, hipStream_t s, bool row_per_query = false);
 // scan = scan_tiles + scan_queries, in one launch (launch_scan_small) for
 // steps of at most kBlock queries.
 // The scans (multi-block query scan for more than kBlock queries, which also
