@@ -1453,9 +1453,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(StepArgs a) {
     // the queries of a tile that spans several, staged in LDS when they fit:
     // all of them, or those with rows when more lie between (queries that
     // skip this step have none; not in the checksum variant, whose registers
-    // it would cost a wave)
+    // it would cost a wave).  The one query of a tile whose source rows are
+    // too many for LDS is staged too, for the sampled search.
     uint32_t nqs = 0;
-    if (qhi - qlo > 1) {
+    if (qhi - qlo > 1 || !staged) {
       const bool all = CK || qhi - qlo <= uint32_t(kQStage);  // then one pass (kQStage = kBlock)
       for (uint32_t q0 = qlo; q0 < qhi && nqs <= uint32_t(kQStage); q0 += kBlock) {
         const uint32_t q = q0 + tid;
