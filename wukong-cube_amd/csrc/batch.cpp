@@ -132,7 +132,7 @@ int begin_batch(wk_dev* dev, const std::vector<Plan>& plans, uint32_t q0, uint32
   StepDesc skip{};
   skip.op = OP_SKIP;
   skip.seg = -1;
-#pragma omp parallel for schedule(static) if (nq >= 1024)
+#pragma omp parallel for schedule(static) num_threads(batch_threads()) if (nq >= 1024)
   for (uint32_t i = 0; i < nq; i++) {
     const Plan& p = plans[q0 + i];
     const uint32_t ns = p.status == WK_OK ? uint32_t(p.steps.size()) : 0;

@@ -294,7 +294,7 @@ void compile_batch(wk_dev* dev, const wk_query* qs, uint32_t nq, const wk_batch_
   const double t0 = host_now();
   std::vector<Plan>& plans = dev->plan_cache;
   if (plans.size() < nq) plans.resize(nq);
-#pragma omp parallel for schedule(static) if (nq >= 1024)
+#pragma omp parallel for schedule(static) num_threads(batch_threads()) if (nq >= 1024)
   for (uint32_t i = 0; i < nq; i++) {
     compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
     out->cols[i] = plans[i].cols;
@@ -972,7 +972,7 @@ int wk_dev_autotune(wk_dev* dev, const wk_query* qs, uint32_t nq, int32_t use_gr
   if (int drc = drain_batches(dev)) return drc;
   std::vector<Plan>& plans = dev->plan_cache;
   if (plans.size() < nq) plans.resize(nq);
-#pragma omp parallel for schedule(static) if (nq >= 1024)
+#pragma omp parallel for schedule(static) num_threads(batch_threads()) if (nq >= 1024)
   for (uint32_t i = 0; i < nq; i++) compile_query(*dev->st, qs[i], plans[i], dev->type_slot.data());
   std::vector<uint64_t> rows(nq), offs(nq);
   std::vector<int32_t> status(nq);

@@ -3,7 +3,9 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <omp.h>
 
+#include <algorithm>
 #include <map>
 #include <utility>
 #include <string>
@@ -16,6 +18,14 @@
 namespace wk {
 
 extern thread_local int g_last_hip_error;
+
+// Threads of the host loops that every batch runs (plan compile, descriptor
+// fill).  OpenMP workers spin between parallel regions, and a batch every few
+// hundred microseconds keeps a full team spinning all the time: with as many
+// threads as the process's CPU quota allows, the quota runs out within its
+// period and the whole process is stopped for milliseconds.  Eight threads do
+// these loops in well under a batch's GPU time and leave the quota room.
+inline int batch_threads() { return std::min(8, omp_get_max_threads()); }
 
 // True when `e` is a failure; the code is kept for wk_last_hip_error().
 inline bool hip_failed(hipError_t e) {
