@@ -23,9 +23,9 @@ extern thread_local int g_last_hip_error;
 // fill).  OpenMP workers spin between parallel regions, and a batch every few
 // hundred microseconds keeps a full team spinning all the time: with as many
 // threads as the process's CPU quota allows, the quota runs out within its
-// period and the whole process is stopped for milliseconds.  Eight threads do
-// these loops in well under a batch's GPU time and leave the quota room.
-inline int batch_threads() { return std::min(8, omp_get_max_threads()); }
+// period and the whole process is stopped for milliseconds.  Four threads
+// fewer (at most 12) leave the runtime's and the caller's threads room.
+inline int batch_threads() { return std::max(1, std::min(12, omp_get_max_threads() - 4)); }
 
 // True when `e` is a failure; the code is kept for wk_last_hip_error().
 inline bool hip_failed(hipError_t e) {
