@@ -773,15 +773,45 @@ __device__ __forceinline__ uint64_t mapped_tiles(const StepArgs& a) {
   return n < a.otile_cap ? n : a.otile_cap;
 }
 
+// The largest g in [0, R) with pfx[g] <= j, found by one wavefront (j is
+// the same in every lane): each round samples 64 evenly spaced rows of the
+// remaining range, one per lane, and keeps the span after the last sample
+// <= j, so R rows take ceil(log64 R) rounds of one dependent load each
+// (4 for 16M rows) instead of log2 R.  pfx[0] = 0 <= j, and the samples <= j
+// are a prefix of the lanes, as pfx does not decrease.
+__device__ __forceinline__ uint64_t wave_find_row(const StepArgs& a, uint64_t R, uint64_t j) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  uint64_t lo = 0, len = R;
+  while (len > 1) {
+    const uint64_t step = (len + kWave - 1) / kWave;
+    const uint64_t o = uint64_t(lane) * step;
+    const bool le = o < len && pfx_at(a, lo + o) <= j;
+    const uint64_t m = __ballot(le);
+    const uint64_t at = uint64_t(63 - __builtin_clzll(m)) * step;  // lane 0's sample is lo itself: m != 0
+    lo += at;
+    len = len - at < step ? len - at : step;
+  }
+  return lo;
+}
+
 // Output tile -> source row of its first output row: the largest g with
 // pfx[g] <= t * kOutTile (a zero-count row can never be that largest g, as
-// the row after it has the same prefix).  One thread per output tile, so a
-// single row with millions of edges costs nothing extra.
-__global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
-  if (a.st->status) return;
-  const uint64_t R = step_rows(a);
-  const uint64_t nt = mapped_tiles(a);
-  for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < nt; t += uint64_t(gridDim.x) * kBlock) {
+// the row after it has the same prefix), for the nt mapped tiles, by the
+// threads of blocks [b0, gridDim.x).  Up to two tiles per wavefront of those
+// blocks, each tile takes a wavefront (a few dependent loads); more tiles take
+// a thread each, by binary search (fewer loads in all).  Either way a single
+// row with millions of edges costs nothing extra.
+__device__ __forceinline__ void mark_tiles(const StepArgs& a, uint64_t R, uint64_t nt, uint32_t b0) {
+  constexpr uint32_t kWaves = kBlock / kWave;
+  const uint64_t nb = gridDim.x - b0, b = blockIdx.x - b0;
+  if (nt <= 2 * nb * kWaves) {
+    for (uint64_t t = b * kWaves + threadIdx.x / kWave; t < nt; t += nb * kWaves) {
+      const uint64_t g = wave_find_row(a, R, a.win_lo + t * kOutTile);
+      if ((threadIdx.x & (kWave - 1)) == 0) a.tile_row[t] = g;
+    }
+    return;
+  }
+  for (uint64_t t = b * kBlock + threadIdx.x; t < nt; t += nb * kBlock) {
     const uint64_t j = a.win_lo + t * kOutTile;
     uint64_t lo = 0, hi = R;
     while (hi - lo > 1) {
@@ -791,6 +821,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
     }
     a.tile_row[t] = lo;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
+  if (a.st->status) return;
+  mark_tiles(a, step_rows(a), mapped_tiles(a), 0);
 }
 
 // Elements of the result arena left for this step's final tables.  A final
@@ -1012,18 +1047,7 @@ __global__ __launch_bounds__(kBlock) void k_query_sums(StepArgs a, uint32_t nqb)
     }
     return;
   }
-  const uint64_t nt = mapped_tiles(a);
-  const uint64_t nb = gridDim.x - nqb;
-  for (uint64_t t = uint64_t(blockIdx.x - nqb) * kBlock + threadIdx.x; t < nt; t += nb * kBlock) {
-    const uint64_t j = a.win_lo + t * kOutTile;
-    uint64_t lo = 0, hi = R;
-    while (hi - lo > 1) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (pfx_at(a, mid) <= j) lo = mid;
-      else hi = mid;
-    }
-    a.tile_row[t] = lo;
-  }
+  mark_tiles(a, R, mapped_tiles(a), nqb);
 }
 
 // Second launch: each block adds the sums of the blocks before it (and the
@@ -2019,7 +2043,7 @@ void launch_scan_small(const StepArgs& a, hipStream_t s) {
 void launch_scan_queries_mb(const StepArgs& a, hipStream_t s) {
   const uint32_t nqb = (a.nq + kBlock - 1) / kBlock;
   // blocks past nqb do k_tile_rows' work
-  const unsigned extra = g_grids.query_sums / 4 > 64 ? g_grids.query_sums / 4 : 64;  // at least one
+  const unsigned extra = g_grids.query_sums / 2 > 64 ? g_grids.query_sums / 2 : 64;
   WK_LAUNCH(k_query_sums, dim3(nqb + extra), dim3(kBlock), 0, s, a, nqb);
   WK_LAUNCH(k_scan_queries_mb, dim3(nqb), dim3(kBlock), 0, s, a, nqb);
 }
@@ -2039,7 +2063,8 @@ void launch_scan(const StepArgs& a, hipStream_t s) { launch_scan_small(a, s); }
 
 void launch_tile_rows(const StepArgs& a, uint64_t out_hint, hipStream_t s) {
   const uint64_t tiles = (out_hint + kOutTile - 1) / kOutTile;
-  const unsigned grid = out_hint ? grid_for((tiles + kBlock - 1) / kBlock, 1u << 20) : g_grids.tile_rows;
+  constexpr unsigned kWaves = kBlock / kWave;  // one output tile per wavefront
+  const unsigned grid = out_hint ? grid_for((tiles + kWaves - 1) / kWaves, g_grids.tile_rows) : g_grids.tile_rows;
   WK_LAUNCH(k_tile_rows, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
