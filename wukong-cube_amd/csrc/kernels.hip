@@ -469,6 +469,7 @@ __device__ __forceinline__ void scan_tiles_block(const StepArgs& a) {
   }
   if (threadIdx.x == 0) {
     a.st->total_out_rows = total;
+    a.st->tiles_marked = 0;
     a.tile_sum[nt] = total;
     if (!(R & (kTile - 1))) a.lpfx[R] = 0;  // row R starts tile nt
   }
@@ -823,8 +824,10 @@ __device__ __forceinline__ void mark_tiles(const StepArgs& a, uint64_t R, uint64
   }
 }
 
+// A step whose few output tiles k_scan_small mapped for the whole output
+// (no window) returns at once.
 __global__ __launch_bounds__(kBlock) void k_tile_rows(StepArgs a) {
-  if (a.st->status) return;
+  if (a.st->status || (a.st->tiles_marked && a.win_lo == 0 && a.win_hi == ~uint64_t(0))) return;
   mark_tiles(a, step_rows(a), mapped_tiles(a), 0);
 }
 
@@ -847,6 +850,54 @@ __device__ __forceinline__ uint32_t step_status(const StepArgs& a, uint64_t carr
 // status.  Every thread of the block must call it.
 __device__ __forceinline__ void scan_queries_block(const StepArgs& a) {
   const uint64_t R = step_rows(a);
+  if (a.nq == 1) {
+    // one query (every single query): its offsets are the bases themselves,
+    // written by one thread without the block scans
+    if (threadIdx.x != 0) return;
+    uint64_t r0 = a.row_start[0], r1 = a.row_start[1];
+    r0 = r0 < R ? r0 : R;
+    r1 = r1 < R ? r1 : R;
+    const uint64_t rows = pfx_at(a, r1) - pfx_at(a, r0);
+    StepDesc& d = a.desc[0];
+    const uint32_t oc = d.out_cols;
+    const bool fin = d.final_step != 0;
+    const uint64_t el = rows * oc;
+    const uint64_t tbase = a.st->tout_base, abase = a.st->arena_base;
+    d.out_rows = rows;
+    if (a.ck && fin) a.ck[0] = 0;
+    if (a.next) {
+      a.next[0].in_rows = fin ? 0 : rows;
+      a.next[0].in_off = tbase;
+      a.next_row_start[0] = 0;
+    }
+    uint64_t aend = abase, tt = 0, tn = 0;
+    if (fin) {
+      const bool fits = abase + el <= a.arena_cap;
+      d.out_off = fits ? abase : kNoSpace;
+      if (!fits) d.out_rows = 0;
+      if (a.qres) {
+        a.qres[0] = fits ? rows : 0;
+        a.qres[1] = fits ? abase : kNoSpace;
+      }
+      if (fits) aend = abase + el;
+    } else {
+      d.out_off = tbase;
+      tt = el;
+      tn = rows;
+    }
+    const uint32_t status = step_status(a, tbase + tt);
+    a.st->out_elems = tt;
+    a.st->status = status;
+    if (a.next) {
+      a.next_row_start[1] = tn;
+      if (a.next_st) {
+        a.next_st->arena_base = aend;
+        a.next_st->tout_base = 0;
+        a.next_st->status = status;
+      }
+    }
+    return;
+  }
   // each thread owns a contiguous run of queries: three block scans in all
   const uint32_t per = (a.nq + 1023) / 1024;
   const uint32_t qlo = threadIdx.x * per < a.nq ? threadIdx.x * per : a.nq;
@@ -1006,6 +1057,20 @@ __global__ __launch_bounds__(1024) void k_scan_small(StepArgs a) {
   scan_tiles_block<1024>(a);
   __syncthreads();
   scan_queries_block(a);
+  // a step of at most kSmallTiles output tiles, gathered whole: its output
+  // tile map too, two tiles per wavefront, instead of a k_tile_rows launch
+  constexpr uint64_t kSmallTiles = 2 * (1024 / kWave);
+  if (a.win_lo != 0 || a.win_hi != ~uint64_t(0)) return;
+  __syncthreads();
+  if (a.st->status) return;
+  const uint64_t nto = mapped_tiles(a);
+  if (nto > kSmallTiles) return;
+  const uint64_t R = step_rows(a);
+  for (uint64_t t = threadIdx.x / kWave; t < nto; t += 1024 / kWave) {
+    const uint64_t g = wave_find_row(a, R, t * kOutTile);
+    if ((threadIdx.x & (kWave - 1)) == 0) a.tile_row[t] = g;
+  }
+  if (threadIdx.x == 0) a.st->tiles_marked = 1;
 }
 
 // Multi-block form of k_scan_queries for large batches.  One block scanning

@@ -88,7 +88,7 @@ struct StepState {
   uint64_t out_elems;       // elements written to the out table (set by scan_queries)
   uint64_t arena_base;      // first free element of the result arena
   uint32_t status;          // 0 ok, 1 table overflow
-  uint32_t pad;
+  uint32_t tiles_marked;    // 1: k_scan_small wrote the output-tile map of the whole step
 };
 
 // Pointers and capacities used by every step kernel.
