@@ -183,6 +183,39 @@ def test_batch_tiles_spanning_more_queries_than_a_block():
     dev.close()
 
 
+@pytest.mark.parametrize("n,frac", [(2000, 0.3), (6000, 0.05)])
+def test_batch_type_filter_steps(univ10_dev, univ10_oracle, univ10_store, n, frac):
+    """Batched type-filter steps (the type bit of each row).  Queries ?X
+    takesCourse C . ?X type GraduateStudent, with C a graduate course for a
+    fraction of them and a department (no rows) for the rest: at 5% a
+    1,024-row probe tile of the filter step spans more than 1,024 queries (row
+    starts searched in memory), at 30% fewer (staged in LDS).  Then the same
+    queries with every tenth one's second pattern an expansion (advisor), so
+    that the step mixes filters and expansions."""
+    U = Q.U
+    courses = np.sort(univ10_store.get_index(U("ub:GraduateCourse"), Q.IN))
+    depts = np.sort(univ10_store.get_index(U("ub:Department"), Q.IN))
+    filt = [(-1, U("ub:takesCourse"), None, Q.OUT), (-1, Q.TYPE_ID, U("ub:GraduateStudent"), Q.OUT)]
+    expand = [(-1, U("ub:takesCourse"), None, Q.OUT), (-1, U("ub:advisor"), -2, Q.OUT)]
+    rng = np.random.default_rng(43)
+    consts = [int(courses[rng.integers(len(courses))]) if rng.random() < frac else int(depts[rng.integers(len(depts))])
+              for _ in range(n)]
+    plan = [(1, "<"), (2, ">")]
+    only = [(Q.set_plan(Q.instantiate(filt, c), plan), 1) for c in consts]
+    mixed = [(Q.set_plan(Q.instantiate(expand if i % 10 == 0 else filt, c), plan), 2 if i % 10 == 0 else 1)
+             for i, c in enumerate(consts)]
+    for qs in (only, mixed):
+        nonempty = 0
+        for use_graph in (True, False):
+            for (pats, nv), (t, v2c, rc) in zip(qs, univ10_dev.batch(qs, use_graph=use_graph)):
+                want, wv2c, wrc = univ10_oracle.query(pats, nv)
+                assert rc == wrc == 0, pats
+                assert v2c == wv2c
+                assert same_rows(t, want), pats
+                nonempty += len(want) > 0
+        assert nonempty > 0
+
+
 def test_batch_overflow_splits(univ10_store, univ10_oracle):
     """A batch whose step table exceeds the table capacity is split and still
     exact; a query that alone exceeds it reports WK_E_NOMEM_BUDGET."""
