@@ -10,12 +10,14 @@ namespace wk {
 
 namespace {
 
-// Pattern steps per captured graph.  The runtime hands a graph's kernels to
-// the GPU in chunks of about 15; a single graph of a 5-step batch (30
-// kernels) showed a gap of up to 90 us before its 16th kernel, every batch.
-// Graphs of two steps (12 kernels) launched back to back have none: 0.59 ms
-// per 4,096-query batch against 0.57-0.66 ms, one step per graph 0.60 ms.
-constexpr uint32_t kStepsPerGraph = 2;
+// Pattern steps per captured graph.  With six kernels per step, a single
+// graph of a 5-step batch (30 kernels) showed a gap of up to 90 us before its
+// 16th kernel in every batch, and graphs of two steps were used.  With five
+// kernels per step, one graph of the 5-step batch (25 kernels) is 1-4%
+// faster than three graphs of at most two steps (0.280-0.284 against
+// 0.284-0.287 ms per pipelined 4,096-query batch; 0.473 against 0.483-0.494
+// ms one at a time), and no slower on the 8-step power-law stream.
+constexpr uint32_t kStepsPerGraph = 5;
 
 void launch_batch_steps(wk_dev* dev, uint32_t k0, uint32_t k1, uint32_t S, uint32_t nq, bool timed) {
   for (uint32_t k = k0; k < k1; k++) {
